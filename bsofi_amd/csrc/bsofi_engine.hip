@@ -1,0 +1,921 @@
+// Planner, executor and native C API of libbsofi_amd.
+//
+// A plan is the whole BSOF/I of a batch of device-resident matrices laid
+// out as an ordered list of kernel launches.  Each launch reads task and
+// work-item tables from one device blob, so matrices of different n and p
+// share launches: step s of the BSOF sweep of every matrix goes into the
+// same pair of launches, row s of every BSTRI sweep into the same GEMM
+// launch, and all BSOI row tiles into one scan launch.  The host/device load
+// balancing of the reference (src/bsofi_hybrid_load_balancing.c, KAPPA_* and
+// *_SWITCH_* of include/bsofi_params.h) has no counterpart: every matrix is
+// device-resident and the plan is the schedule.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "bsofi.h"
+#include "bsofi_internal.h"
+#include "bsofi_kernels.hip"
+
+namespace bsofi {
+
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  static int reported = 0;
+  if (reported < 8) {
+    fprintf(stderr, "bsofi_amd: %s failed: %s\n", what, hipGetErrorString(e));
+    reported++;
+  }
+  return false;
+}
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    if (!hip_ok((x), #x)) return BSOFI_ERR_NO_DEVICE; \
+  } while (0)
+
+enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCAN = 5, F_LARFT = 6, F_FILL = 7 };
+
+struct Launch {
+  int family = 0, variant = 0, grid = 0, ntasks = 0, ldcar = 0;
+  size_t task_off = 0, item_off = 0, shmem = 0;
+};
+
+struct Blob {
+  std::vector<char> h;
+  template <class T>
+  size_t push(const std::vector<T>& v) {
+    size_t off = (h.size() + 255) & ~(size_t)255;
+    h.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) memcpy(h.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+// Per-matrix geometry of the blocked algorithm.
+struct Geo {
+  int n, p, lda, N, nb, S;
+  int panel_variant;  // panel_factor_kernel instance
+  int scan_mti, scan_ntw;
+};
+
+static int nb_for(int n) { return (2 * n <= 512) ? 32 : (2 * n <= 1024 ? 16 : 8); }
+
+static Geo geometry(int n, int p, int lda) {
+  Geo g;
+  g.n = n;
+  g.p = p;
+  g.lda = lda;
+  g.N = n * p;
+  g.nb = nb_for(n);
+  g.S = (2 * n + g.nb - 1) / g.nb;
+  const int m = 2 * n;
+  g.panel_variant = m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : m <= 1024 ? 3 : 4;
+  g.scan_mti = n <= 256 ? 2 : 1;
+  const int tiles = (2 * n + 15) / 16;
+  g.scan_ntw = (tiles + 3) / 4;
+  return g;
+}
+
+// Workspace layout of one matrix (doubles, each piece 256-byte aligned).
+struct WsLayout {
+  size_t T, Q, W, Z1, Z2, Y, total;  // byte offsets
+};
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static WsLayout ws_layout(const Geo& g, int phases) {
+  WsLayout L{};
+  size_t off = 0;
+  const size_t n2 = (size_t)g.n * g.n;
+  auto take = [&](size_t doubles) {
+    size_t o = off;
+    off += al256(doubles * sizeof(double));
+    return o;
+  };
+  const bool need_T = phases & (BSOFI_PHASE_BSOF | BSOFI_PHASE_BSOI);
+  L.T = need_T ? take((size_t)(g.p - 1) * g.S * g.nb * g.nb) : 0;
+  L.Q = (phases & BSOFI_PHASE_BSOI) ? take((size_t)(g.p - 1) * 4 * n2) : 0;
+  if (phases & BSOFI_PHASE_BSTRI) {
+    L.W = take((size_t)(g.p - 1) * n2);
+    L.Z1 = take((size_t)std::max(g.p - 2, 1) * n2);
+    L.Z2 = take((size_t)std::max(g.p - 2, 1) * n2);
+    L.Y = take((size_t)g.p * g.n * 32);
+  }
+  L.total = off;
+  return L;
+}
+
+}  // namespace bsofi
+
+using namespace bsofi;
+
+struct bsofi_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+struct bsofi_plan_s {
+  bsofi_ctx_s* ctx = nullptr;
+  std::vector<Launch> launches;
+  Blob blob;
+  char* d_blob = nullptr;
+  char* ws = nullptr;
+  bool own_ws = false;
+  size_t ws_bytes = 0;
+  int* d_info = nullptr;
+  int count = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int timed_family = -1;
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;
+};
+
+namespace bsofi {
+
+// ---------------------------------------------------------------------------
+// Planner
+// ---------------------------------------------------------------------------
+struct MatJob {
+  Geo g;
+  double* A;
+  double* tau;
+  int* info;
+  double *T, *Q, *W, *Z1, *Z2, *Y;
+};
+
+static double* blk(const MatJob& m, int i, int j) {
+  return m.A + (size_t)m.g.n * ((size_t)j * m.g.lda + i);
+}
+
+static double* Tptr(const MatJob& m, int k, int j) {
+  return m.T + ((size_t)k * m.g.S + j) * m.g.nb * m.g.nb;
+}
+
+struct Planner {
+  Blob& blob;
+  std::vector<Launch>& out;
+  Planner(Blob& b, std::vector<Launch>& o) : blob(b), out(o) {}
+
+  template <class T>
+  void emit(int family, int variant, const std::vector<T>& tasks, const std::vector<Item>& items, int ldcar = 0,
+            size_t shmem = 0) {
+    if (tasks.empty() || items.empty()) return;
+    Launch L;
+    L.family = family;
+    L.variant = variant;
+    L.task_off = blob.push(tasks);
+    L.item_off = blob.push(items);
+    L.ntasks = (int)tasks.size();
+    L.grid = (int)items.size();
+    L.ldcar = ldcar;
+    L.shmem = shmem;
+    out.push_back(L);
+  }
+
+  // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------
+  void bsof(std::vector<MatJob>& mats) {
+    struct Step { int k, j; };
+    std::vector<std::vector<Step>> seq(mats.size());
+    size_t rounds = 0;
+    for (size_t i = 0; i < mats.size(); ++i) {
+      const Geo& g = mats[i].g;
+      for (int k = 0; k <= g.p - 2; ++k) {
+        const int ncols = (k == g.p - 2) ? 2 * g.n : g.n;
+        for (int j = 0; j * g.nb < ncols; ++j) seq[i].push_back({k, j});
+      }
+      rounds = std::max(rounds, seq[i].size());
+    }
+    for (size_t r = 0; r < rounds; ++r) {
+      std::vector<PanelTask> pt[5];
+      std::vector<Item> pi[5];
+      std::vector<ReflTask> rt;
+      std::vector<Item> ri;
+      for (size_t i = 0; i < mats.size(); ++i) {
+        if (r >= seq[i].size()) continue;
+        const MatJob& M = mats[i];
+        const Geo& g = M.g;
+        const int k = seq[i][r].k, j = seq[i][r].j;
+        const int ncols = (k == g.p - 2) ? 2 * g.n : g.n;
+        const int off = k * g.n + j * g.nb;  // row == col of the sub-panel corner
+        const int m = 2 * g.n - j * g.nb;
+        const int nbj = std::min(g.nb, ncols - j * g.nb);
+        double* P = M.A + (size_t)off * g.lda + off;
+        PanelTask t{P, M.tau + off, Tptr(M, k, j), M.info, g.lda, m, nbj, g.nb};
+        const int v = g.panel_variant;
+        pi[v].push_back({(int)pt[v].size(), 0});
+        pt[v].push_back(t);
+        // trailing columns of the panel + target 1 (block column k+1) are
+        // contiguous; target 2 is the last block column.
+        const int w1 = 2 * g.n - j * g.nb - nbj;
+        const int w2 = (k < g.p - 2) ? g.n : 0;
+        if (w1 + w2 > 0) {
+          ReflTask q;
+          q.V = P;
+          q.T = Tptr(M, k, j);
+          q.C1 = M.A + (size_t)(off + nbj) * g.lda + off;
+          q.C2 = w2 ? M.A + (size_t)((g.p - 1) * g.n) * g.lda + off : nullptr;
+          q.ldv = g.lda;
+          q.ldt = g.nb;
+          q.ldc = g.lda;
+          q.m = m;
+          q.nbj = nbj;
+          q.w1 = w1;
+          q.w2 = w2;
+          q.trans = 1;
+          const int slabs = (w1 + w2 + 15) / 16;
+          for (int s = 0; s < slabs; ++s) ri.push_back({(int)rt.size(), s});
+          rt.push_back(q);
+        }
+      }
+      for (int v = 0; v < 5; ++v) emit(F_PANEL, v, pt[v], pi[v]);
+      emit(F_REFL, 0, rt, ri);
+    }
+  }
+
+  // -- BSTRI: src/bsofi_cpu.c:146-177, restructured by block rows ---------
+  void gemm_items(std::vector<GemmTask>& tasks, std::vector<Item>& items, GemmTask t) {
+    if (t.M <= 0 || t.N <= 0) return;
+    t.tiles_n = (t.N + 63) / 64;
+    const int tm = (t.M + 63) / 64;
+    const int id = (int)tasks.size();
+    tasks.push_back(t);
+    for (int s = 0; s < tm * t.tiles_n; ++s) items.push_back({id, s});
+  }
+
+  static GemmTask gt(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
+                     double alpha, int amask, int bmask) {
+    GemmTask t{};
+    t.A = A;
+    t.B = B;
+    t.C = C;
+    t.D = nullptr;
+    t.alpha = alpha;
+    t.beta = 0.0;
+    t.lda = lda;
+    t.ldb = ldb;
+    t.ldc = ldc;
+    t.ldd = 0;
+    t.M = M;
+    t.N = N;
+    t.K = K;
+    t.amask = amask;
+    t.bmask = bmask;
+    t.dcol0 = INT_MAX;
+    t.tiles_n = 1;
+    return t;
+  }
+
+  void bstri(std::vector<MatJob>& mats) {
+    // (1) X_ii = R_ii^{-1} for every diagonal block: 32 x 32 diagonal
+    // sub-blocks first, then block columns J = 1.. by two GEMMs each:
+    //   Y = R_{0:J,J} X_JJ,   X_{0:J,J} = -X_{0:J,0:J} Y.
+    std::vector<TrtriTask> tt;
+    int maxJ = 0;
+    for (auto& M : mats) {
+      const Geo& g = M.g;
+      const int nsb = (g.n + 31) / 32;
+      maxJ = std::max(maxJ, nsb);
+      for (int i = 0; i < g.p; ++i)
+        for (int b = 0; b < nsb; ++b) {
+          const int o = i * g.n + 32 * b;
+          tt.push_back({M.A + (size_t)o * g.lda + o, M.info, g.lda, std::min(32, g.n - 32 * b), o + 1, 0});
+        }
+    }
+    if (!tt.empty()) {
+      Launch L;
+      L.family = F_TRTRI;
+      L.task_off = blob.push(tt);
+      L.ntasks = (int)tt.size();
+      L.grid = (int)tt.size();
+      out.push_back(L);
+    }
+    for (int J = 1; J < maxJ; ++J) {
+      std::vector<GemmTask> t1, t2;
+      std::vector<Item> i1, i2;
+      for (auto& M : mats) {
+        const Geo& g = M.g;
+        if (32 * J >= g.n) continue;
+        const int nbJ = std::min(32, g.n - 32 * J);
+        for (int i = 0; i < g.p; ++i) {
+          double* Bii = blk(M, i, i);
+          double* Yi = M.Y + (size_t)i * g.n * 32;
+          gemm_items(t1, i1, gt(Bii + (size_t)32 * J * g.lda, g.lda, Bii + (size_t)32 * J * (g.lda + 1), g.lda, Yi,
+                                g.n, 32 * J, nbJ, nbJ, 1.0, 0, 1));
+          gemm_items(t2, i2, gt(Bii, g.lda, Yi, g.n, Bii + (size_t)32 * J * g.lda, g.lda, 32 * J, nbJ, 32 * J, -1.0, 1,
+                                0));
+        }
+      }
+      emit(F_GEMM, 0, t1, i1);
+      emit(F_GEMM, 0, t2, i2);
+    }
+    // (2) W_i = -X_ii R_{i,i+1} (i <= p-2) and Z1_i = R_{i,p-1} X_{p-1,p-1}
+    // (i <= p-3), then Z2_i = -X_ii Z1_i: the two TRMMs of
+    // src/bsofi_cpu.c:160,163,165 applied to every row at once.
+    {
+      std::vector<GemmTask> t1, t2;
+      std::vector<Item> i1, i2;
+      for (auto& M : mats) {
+        const Geo& g = M.g;
+        const size_t n2 = (size_t)g.n * g.n;
+        for (int i = 0; i <= g.p - 2; ++i)
+          gemm_items(t1, i1, gt(blk(M, i, i), g.lda, blk(M, i, i + 1), g.lda, M.W + i * n2, g.n, g.n, g.n, g.n, -1.0, 1,
+                                0));
+        for (int i = 0; i <= g.p - 3; ++i) {
+          gemm_items(t1, i1, gt(blk(M, i, g.p - 1), g.lda, blk(M, g.p - 1, g.p - 1), g.lda, M.Z1 + i * n2, g.n, g.n,
+                                g.n, g.n, 1.0, 0, 1));
+          gemm_items(t2, i2, gt(blk(M, i, i), g.lda, M.Z1 + i * n2, g.n, M.Z2 + i * n2, g.n, g.n, g.n, g.n, -1.0, 1,
+                                0));
+        }
+      }
+      emit(F_GEMM, 0, t1, i1);
+      emit(F_GEMM, 0, t2, i2);
+    }
+    // (3) Row sweep, bottom-up: X_{i,i+1:p-1} = W_i X_{i+1,i+1:p-1}, plus
+    // Z2_i on the last block column (src/bsofi_cpu.c:161-167).
+    int rounds = 0;
+    for (auto& M : mats) rounds = std::max(rounds, M.g.p - 1);
+    for (int s = 0; s < rounds; ++s) {
+      std::vector<GemmTask> t;
+      std::vector<Item> it;
+      for (auto& M : mats) {
+        const Geo& g = M.g;
+        const int i = g.p - 2 - s;
+        if (i < 0) continue;
+        GemmTask q = gt(M.W + (size_t)i * g.n * g.n, g.n, blk(M, i + 1, i + 1), g.lda, blk(M, i, i + 1), g.lda, g.n,
+                        (g.p - 1 - i) * g.n, g.n, 1.0, 0, 1);
+        if (i <= g.p - 3) {
+          q.D = M.Z2 + (size_t)i * g.n * g.n;
+          q.ldd = g.n;
+          q.dcol0 = (g.p - 2 - i) * g.n;
+        }
+        gemm_items(t, it, q);
+      }
+      emit(F_GEMM, 0, t, it);
+    }
+  }
+
+  // -- BSOI: src/bsofi_cpu.c:283-333 ----------------------------------------
+  void bsoi(std::vector<MatJob>& mats, int max_shmem) {
+    // (1) T factors from (V, tau), (2) Q_k = I, (3) Q_k <- B_j Q_k for
+    // j = last .. 0 (dorgqr's backward accumulation), (4) row scans.
+    std::vector<PanelTask> lt;
+    std::vector<Item> li;
+    std::vector<EyeTask> et;
+    int maxS = 0;
+    for (auto& M : mats) {
+      const Geo& g = M.g;
+      for (int k = 0; k <= g.p - 2; ++k) {
+        const int ncols = (k == g.p - 2) ? 2 * g.n : g.n;
+        for (int j = 0; j * g.nb < ncols; ++j) {
+          const int off = k * g.n + j * g.nb;
+          PanelTask t{M.A + (size_t)off * g.lda + off, M.tau + off, Tptr(M, k, j), M.info, g.lda,
+                      2 * g.n - j * g.nb, std::min(g.nb, ncols - j * g.nb), g.nb};
+          li.push_back({(int)lt.size(), 0});
+          lt.push_back(t);
+        }
+      }
+      maxS = std::max(maxS, g.S);
+      et.push_back({M.Q, (long long)4 * g.n * g.n, 2 * g.n, g.p - 1});
+    }
+    emit(F_LARFT, 0, lt, li);
+    if (!et.empty()) {
+      Launch L;
+      L.family = F_EYE;
+      L.task_off = blob.push(et);
+      L.ntasks = (int)et.size();
+      L.grid = 1024;
+      out.push_back(L);
+    }
+    for (int r = 0; r < maxS; ++r) {
+      std::vector<ReflTask> rt;
+      std::vector<Item> ri;
+      for (auto& M : mats) {
+        const Geo& g = M.g;
+        for (int k = 0; k <= g.p - 2; ++k) {
+          const int ncols = (k == g.p - 2) ? 2 * g.n : g.n;
+          const int S = (ncols + g.nb - 1) / g.nb;
+          const int j = S - 1 - r;
+          if (j < 0) continue;
+          const int off = k * g.n + j * g.nb;
+          const int n2 = 2 * g.n;
+          ReflTask q;
+          q.V = M.A + (size_t)off * g.lda + off;
+          q.T = Tptr(M, k, j);
+          double* Qk = M.Q + (size_t)k * 4 * g.n * g.n;
+          q.C1 = Qk + (size_t)(j * g.nb) * n2 + j * g.nb;
+          q.C2 = nullptr;
+          q.ldv = g.lda;
+          q.ldt = g.nb;
+          q.ldc = n2;
+          q.m = n2 - j * g.nb;
+          q.nbj = std::min(g.nb, ncols - j * g.nb);
+          q.w1 = n2 - j * g.nb;
+          q.w2 = 0;
+          q.trans = 0;
+          const int slabs = (q.w1 + 15) / 16;
+          for (int s = 0; s < slabs; ++s) ri.push_back({(int)rt.size(), s});
+          rt.push_back(q);
+        }
+      }
+      emit(F_REFL, 0, rt, ri);
+    }
+    // row scans, grouped by kernel instance; heaviest tiles first
+    struct Var { int mti, ntw; };
+    std::vector<std::pair<int, int>> keys;
+    for (auto& M : mats) keys.push_back({M.g.scan_mti, M.g.scan_ntw});
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (auto key : keys) {
+      std::vector<ScanTask> st;
+      std::vector<Item> si;
+      int maxn = 0;
+      struct W { double cost; int task, sub; };
+      std::vector<W> order;
+      for (auto& M : mats) {
+        const Geo& g = M.g;
+        if (g.scan_mti != key.first || g.scan_ntw != key.second) continue;
+        const int MT = 16 * g.scan_mti;
+        const int id = (int)st.size();
+        st.push_back({M.A, M.Q, (long long)4 * g.n * g.n, g.lda, g.n, g.p, 0, g.N, 0});
+        maxn = std::max(maxn, g.n);
+        const int tiles = (g.N + MT - 1) / MT;
+        for (int s = 0; s < tiles; ++s) {
+          const int rb = (s * MT) / g.n;
+          const double cost = (double)g.n * g.n * (2.0 * std::max(0, g.p - 1 - rb) + std::min(rb, g.p - 1));
+          order.push_back({cost, id, s});
+        }
+      }
+      std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.cost > b.cost; });
+      for (auto& w : order) si.push_back({w.task, w.sub});
+      const int MT = 16 * key.first;
+      int ldcar = MT + 1;
+      if ((size_t)maxn * ldcar * sizeof(double) > (size_t)max_shmem) ldcar = MT;
+      emit(F_SCAN, key.first * 100 + key.second, st, si, ldcar, (size_t)maxn * ldcar * sizeof(double));
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Executor
+// ---------------------------------------------------------------------------
+template <int MTI, int NTW>
+static void launch_scan(const Launch& L, const char* base, hipStream_t s) {
+  hipLaunchKernelGGL((bsoi_scan_kernel<MTI, NTW>), dim3(L.grid), dim3(256), L.shmem, s,
+                     (const ScanTask*)(base + L.task_off), (const Item*)(base + L.item_off), L.ldcar);
+}
+
+template <int MTI, int NTW>
+static void set_scan_attr(size_t shmem) {
+  if (shmem > 65536)
+    (void)hipFuncSetAttribute((const void*)bsoi_scan_kernel<MTI, NTW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shmem);
+}
+
+#define SCAN_SWITCH(FN, ...)                                  \
+  switch (variant) {                                          \
+    case 201: FN<2, 1>(__VA_ARGS__); break;                   \
+    case 202: FN<2, 2>(__VA_ARGS__); break;                   \
+    case 203: FN<2, 3>(__VA_ARGS__); break;                   \
+    case 204: FN<2, 4>(__VA_ARGS__); break;                   \
+    case 205: FN<2, 5>(__VA_ARGS__); break;                   \
+    case 206: FN<2, 6>(__VA_ARGS__); break;                   \
+    case 207: FN<2, 7>(__VA_ARGS__); break;                   \
+    case 208: FN<2, 8>(__VA_ARGS__); break;                   \
+    case 109: FN<1, 9>(__VA_ARGS__); break;                   \
+    case 110: FN<1, 10>(__VA_ARGS__); break;                  \
+    case 111: FN<1, 11>(__VA_ARGS__); break;                  \
+    case 112: FN<1, 12>(__VA_ARGS__); break;                  \
+    case 113: FN<1, 13>(__VA_ARGS__); break;                  \
+    case 114: FN<1, 14>(__VA_ARGS__); break;                  \
+    case 115: FN<1, 15>(__VA_ARGS__); break;                  \
+    case 116: FN<1, 16>(__VA_ARGS__); break;                  \
+    default: break;                                           \
+  }
+
+static void run_launch(const Launch& L, const char* base, hipStream_t s) {
+  const int variant = L.variant;
+  switch (L.family) {
+    case F_PANEL: {
+      auto t = (const PanelTask*)(base + L.task_off);
+      auto it = (const Item*)(base + L.item_off);
+      switch (L.variant) {
+        case 0: hipLaunchKernelGGL((panel_factor_kernel<2, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 1: hipLaunchKernelGGL((panel_factor_kernel<4, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 2: hipLaunchKernelGGL((panel_factor_kernel<8, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 3: hipLaunchKernelGGL((panel_factor_kernel<16, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 4: hipLaunchKernelGGL((panel_factor_kernel<32, 2>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+      }
+      break;
+    }
+    case F_REFL:
+      hipLaunchKernelGGL(refl_apply_kernel, dim3(L.grid), dim3(256), 0, s, (const ReflTask*)(base + L.task_off),
+                         (const Item*)(base + L.item_off));
+      break;
+    case F_GEMM:
+      hipLaunchKernelGGL(gemm_kernel, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
+                         (const Item*)(base + L.item_off));
+      break;
+    case F_TRTRI:
+      hipLaunchKernelGGL(trtri_diag_kernel, dim3(L.grid), dim3(64), 0, s, (const TrtriTask*)(base + L.task_off));
+      break;
+    case F_EYE:
+      hipLaunchKernelGGL(eye_kernel, dim3(L.grid), dim3(256), 0, s, (const EyeTask*)(base + L.task_off), L.ntasks);
+      break;
+    case F_LARFT:
+      hipLaunchKernelGGL(larft_kernel, dim3(L.grid), dim3(256), 0, s, (const PanelTask*)(base + L.task_off),
+                         (const Item*)(base + L.item_off));
+      break;
+    case F_SCAN: {
+      // variants outside the instance table are rejected at plan time
+      SCAN_SWITCH(launch_scan, L, base, s)
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+static int scan_variant_ok(int mti, int ntw) {
+  if (mti == 2) return ntw >= 1 && ntw <= 8;
+  if (mti == 1) return ntw >= 9 && ntw <= 16;
+  return 0;
+}
+
+}  // namespace bsofi
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+int bsofi_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int bsofi_check_args(int n, int L, int lda) {
+  if (n < 1) return -1;
+  if (L < 2) return -2;
+  if ((long long)lda < (long long)n * L) return -4;
+  if (n > BSOFI_MAX_N) return BSOFI_ERR_UNSUPPORTED;
+  return 0;
+}
+
+int bsofi_ctx_create(bsofi_ctx_t* ctx, int device, void* hip_stream) {
+  if (!ctx) return -1;
+  *ctx = nullptr;
+  int count = bsofi_device_count();
+  if (count <= 0 || device < 0 || device >= count) return BSOFI_ERR_NO_DEVICE;
+  HIPCHK(hipSetDevice(device));
+  auto* c = new bsofi_ctx_s();
+  c->device = device;
+  if (hip_stream) {
+    c->stream = (hipStream_t)hip_stream;
+  } else {
+    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+      delete c;
+      return BSOFI_ERR_NO_DEVICE;
+    }
+    c->own_stream = true;
+  }
+  *ctx = c;
+  return 0;
+}
+
+int bsofi_ctx_destroy(bsofi_ctx_t c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->own_stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return 0;
+}
+
+int bsofi_ctx_set_stream(bsofi_ctx_t c, void* s) {
+  if (!c) return -1;
+  if (c->own_stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    c->own_stream = false;
+  }
+  c->stream = (hipStream_t)s;
+  return 0;
+}
+
+void* bsofi_ctx_stream(bsofi_ctx_t c) { return c ? (void*)c->stream : nullptr; }
+
+int bsofi_ctx_synchronize(bsofi_ctx_t c) {
+  if (!c) return -1;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+size_t bsofi_workspace_bytes(int n, int p) {
+  if (bsofi_check_args(n, p, n * p) != 0) return 0;
+  return ws_layout(geometry(n, p, n * p), BSOFI_PHASE_ALL).total;
+}
+
+size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int phases) {
+  size_t total = al256((size_t)count * sizeof(int));
+  for (int i = 0; i < count; ++i) total += ws_layout(geometry(mats[i].n, mats[i].p, mats[i].lda), phases).total;
+  return total;
+}
+
+int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, void* workspace,
+                      bsofi_plan_t* out) {
+  if (!ctx || !out || count < 0 || (count > 0 && !mats)) return -1;
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) {
+    int a = bsofi_check_args(mats[i].n, mats[i].p, mats[i].lda);
+    if (a) return a;
+    if (!mats[i].A || !mats[i].tau) return -3;
+    const Geo g = geometry(mats[i].n, mats[i].p, mats[i].lda);
+    if (!scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* P = new bsofi_plan_s();
+  P->ctx = ctx;
+  P->count = count;
+  P->ws_bytes = bsofi_plan_workspace_bytes(count, mats, phases);
+  if (workspace) {
+    P->ws = (char*)workspace;
+  } else if (P->ws_bytes) {
+    if (hipMalloc((void**)&P->ws, P->ws_bytes) != hipSuccess) {
+      delete P;
+      return BSOFI_ERR_ALLOC;
+    }
+    P->own_ws = true;
+  }
+  P->d_info = (int*)P->ws;
+  std::vector<MatJob> jobs(count);
+  size_t off = al256((size_t)count * sizeof(int));
+  for (int i = 0; i < count; ++i) {
+    MatJob& J = jobs[i];
+    J.g = geometry(mats[i].n, mats[i].p, mats[i].lda);
+    J.A = mats[i].A;
+    J.tau = mats[i].tau;
+    J.info = P->d_info + i;
+    const WsLayout L = ws_layout(J.g, phases);
+    char* b = P->ws + off;
+    J.T = (double*)(b + L.T);
+    J.Q = (double*)(b + L.Q);
+    J.W = (double*)(b + L.W);
+    J.Z1 = (double*)(b + L.Z1);
+    J.Z2 = (double*)(b + L.Z2);
+    J.Y = (double*)(b + L.Y);
+    off += L.total;
+  }
+  int max_shmem = 65536;
+  (void)hipDeviceGetAttribute(&max_shmem, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+  Planner pl(P->blob, P->launches);
+  if (count > 0) {
+    // info words start at INT_MAX ("no zero pivot seen"); the fill launch
+    // addresses the workspace directly (task_off holds no blob offset).
+    Launch L;
+    L.family = F_FILL;
+    L.ntasks = count;
+    L.grid = 1;
+    P->launches.push_back(L);
+  }
+  if (phases & BSOFI_PHASE_BSOF) pl.bsof(jobs);
+  if (phases & BSOFI_PHASE_BSTRI) pl.bstri(jobs);
+  if (phases & BSOFI_PHASE_BSOI) pl.bsoi(jobs, max_shmem);
+  for (auto& L : P->launches) {
+    if (L.family == F_SCAN) {
+      const int variant = L.variant;
+      SCAN_SWITCH(set_scan_attr, L.shmem)
+    }
+  }
+  const size_t bytes = std::max<size_t>(P->blob.h.size(), 256);
+  if (hipMalloc((void**)&P->d_blob, bytes) != hipSuccess) {
+    if (P->own_ws) (void)hipFree(P->ws);
+    delete P;
+    return BSOFI_ERR_ALLOC;
+  }
+  if (!P->blob.h.empty() &&
+      !hip_ok(hipMemcpyAsync(P->d_blob, P->blob.h.data(), P->blob.h.size(), hipMemcpyHostToDevice, ctx->stream),
+              "hipMemcpyAsync(plan)")) {
+    (void)hipFree(P->d_blob);
+    if (P->own_ws) (void)hipFree(P->ws);
+    delete P;
+    return BSOFI_ERR_NO_DEVICE;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  *out = P;
+  return 0;
+}
+
+static void run_all(bsofi_plan_s* P) {
+  hipStream_t s = P->ctx->stream;
+  for (const Launch& L : P->launches) {
+    if (L.family == F_FILL) {
+      hipLaunchKernelGGL(fill_int_kernel, dim3((L.ntasks + 255) / 256), dim3(256), 0, s, P->d_info, L.ntasks,
+                         INT_MAX);
+      continue;
+    }
+    const bool timed = (L.family == P->timed_family);
+    if (timed) {
+      if ((size_t)P->ev_used + 2 > P->ev.size()) {
+        for (int k = 0; k < 2; ++k) {
+          hipEvent_t e;
+          (void)hipEventCreate(&e);
+          P->ev.push_back(e);
+        }
+      }
+      (void)hipEventRecord(P->ev[P->ev_used], s);
+    }
+    run_launch(L, P->d_blob, s);
+    if (timed) {
+      (void)hipEventRecord(P->ev[P->ev_used + 1], s);
+      P->ev_used += 2;
+    }
+  }
+}
+
+int bsofi_plan_execute(bsofi_plan_t P) {
+  if (!P) return -1;
+  HIPCHK(hipSetDevice(P->ctx->device));
+  hipStream_t s = P->ctx->stream;
+  if (P->use_graph && P->timed_family < 0) {
+    if (!P->exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      run_all(P);
+      HIPCHK(hipStreamEndCapture(s, &P->graph));
+      HIPCHK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(P->exec, s));
+  } else {
+    run_all(P);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bsofi_plan_use_graph(bsofi_plan_t P, int on) {
+  if (!P) return -1;
+  P->use_graph = on != 0;
+  return 0;
+}
+
+int bsofi_plan_num_launches(bsofi_plan_t P) { return P ? (int)P->launches.size() : -1; }
+
+int bsofi_plan_info(bsofi_plan_t P, int* info) {
+  if (!P || !info) return -1;
+  if (P->count == 0) return 0;
+  HIPCHK(hipSetDevice(P->ctx->device));
+  HIPCHK(hipMemcpyAsync(info, P->d_info, sizeof(int) * P->count, hipMemcpyDeviceToHost, P->ctx->stream));
+  HIPCHK(hipStreamSynchronize(P->ctx->stream));
+  for (int i = 0; i < P->count; ++i)
+    if (info[i] == INT_MAX) info[i] = 0;
+  return 0;
+}
+
+int bsofi_plan_time_family(bsofi_plan_t P, int family) {
+  if (!P) return -1;
+  P->timed_family = family;
+  P->ev_used = 0;
+  return 0;
+}
+
+int bsofi_plan_family_time(bsofi_plan_t P, double* ms, int* launches) {
+  if (!P) return -1;
+  HIPCHK(hipSetDevice(P->ctx->device));
+  HIPCHK(hipStreamSynchronize(P->ctx->stream));
+  double total = 0.0;
+  for (int k = 0; k + 1 < P->ev_used; k += 2) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, P->ev[k], P->ev[k + 1]) == hipSuccess) total += t;
+  }
+  if (ms) *ms = total;
+  if (launches) *launches = P->ev_used / 2;
+  P->ev_used = 0;
+  return 0;
+}
+
+int bsofi_plan_destroy(bsofi_plan_t P) {
+  if (!P) return 0;
+  (void)hipSetDevice(P->ctx->device);
+  (void)hipStreamSynchronize(P->ctx->stream);
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  for (auto e : P->ev) (void)hipEventDestroy(e);
+  if (P->d_blob) (void)hipFree(P->d_blob);
+  if (P->own_ws && P->ws) (void)hipFree(P->ws);
+  delete P;
+  return 0;
+}
+
+int bsofi_invert_batched(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, int* info) {
+  bsofi_plan_t P = nullptr;
+  int rc = bsofi_plan_create(ctx, count, mats, phases, nullptr, &P);
+  if (rc) return rc;
+  rc = bsofi_plan_execute(P);
+  if (rc == 0 && info) rc = bsofi_plan_info(P, info);
+  bsofi_plan_destroy(P);
+  return rc;
+}
+
+int bsofi_gather_blocks(bsofi_ctx_t ctx, const double* dA, int lda, int n, int p, double* dblocks) {
+  if (!ctx) return -1;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(gather_blocks_kernel, dim3(1024), dim3(256), 0, ctx->stream, dA, lda, n, p, dblocks);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bsofi_pcyclic_residual(bsofi_ctx_t ctx, const double* dblocks, const double* dG, int ldg, int n, int p,
+                           double* resid) {
+  if (!ctx || !resid) return -1;
+  HIPCHK(hipSetDevice(ctx->device));
+  double* acc = nullptr;
+  HIPCHK(hipMalloc((void**)&acc, 3 * sizeof(double)));
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, (int*)acc, 6, 0);
+  const int N = n * p;
+  const int tiles = p * ((n + 63) / 64) * ((N + 63) / 64);
+  hipLaunchKernelGGL(pcyclic_residual_kernel, dim3(tiles), dim3(256), 0, ctx->stream, dblocks, dG, ldg, n, p, acc);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(1024), dim3(256), 0, ctx->stream, dblocks, n, n, 2 * n * p, acc + 1);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(1024), dim3(256), 0, ctx->stream, dG, ldg, N, N, acc + 2);
+  double h[3] = {0, 0, 0};
+  hipError_t e = hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(acc);
+  if (!hip_ok(e, "residual")) return BSOFI_ERR_NO_DEVICE;
+  *resid = std::sqrt(h[0]) / (std::sqrt(h[1]) * std::sqrt(h[2]));
+  return 0;
+}
+
+int bsofi_debug_gemm(bsofi_ctx_t ctx, int M, int N, int K, double alpha, const double* dA, int lda,
+                     const double* dB, int ldb, double beta, double* dC, int ldc, int amask, int bmask) {
+  if (!ctx) return -1;
+  if (M < 0 || N < 0 || K < 0) return -2;
+  HIPCHK(hipSetDevice(ctx->device));
+  GemmTask t{};
+  t.A = dA;
+  t.B = dB;
+  t.C = dC;
+  t.D = nullptr;
+  t.alpha = alpha;
+  t.beta = beta;
+  t.lda = lda;
+  t.ldb = ldb;
+  t.ldc = ldc;
+  t.M = M;
+  t.N = N;
+  t.K = K;
+  t.amask = amask;
+  t.bmask = bmask;
+  t.dcol0 = INT_MAX;
+  t.tiles_n = (N + 63) / 64;
+  const int tiles = ((M + 63) / 64) * t.tiles_n;
+  if (tiles == 0) return 0;
+  std::vector<Item> items(tiles);
+  for (int s2 = 0; s2 < tiles; ++s2) items[s2] = {0, s2};
+  char* d = nullptr;
+  const size_t ib = al256(sizeof(GemmTask));
+  HIPCHK(hipMalloc((void**)&d, ib + items.size() * sizeof(Item)));
+  hipError_t e = hipMemcpyAsync(d, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d + ib, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(gemm_kernel, dim3(tiles), dim3(256), 0, ctx->stream, (const GemmTask*)d, (const Item*)(d + ib));
+    e = hipStreamSynchronize(ctx->stream);
+  }
+  (void)hipFree(d);
+  return hip_ok(e, "debug_gemm") ? 0 : BSOFI_ERR_NO_DEVICE;
+}
+
+int bsofi_dqmc_build(bsofi_ctx_t ctx, double* dA, int lda, int n, int p, const double* dexpk,
+                     const signed char* dfields, double ep, double em) {
+  if (!ctx) return -1;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(dqmc_build_kernel, dim3(4096), dim3(256), 0, ctx->stream, dA, lda, n, p, dexpk, dfields, ep,
+                     em);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bsofi_dqmc_flip(bsofi_ctx_t ctx, signed char* dfields, const int* dflips, int count) {
+  if (!ctx) return -1;
+  if (count <= 0) return 0;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(hs_flip_kernel, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, dfields, dflips, count);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

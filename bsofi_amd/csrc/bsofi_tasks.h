@@ -1,0 +1,87 @@
+// Task descriptors shared by the host planner and the gfx950 kernels.
+//
+// The planner (bsofi_engine.cpp) turns a batch of p-cyclic matrices into an
+// ordered list of launches.  Every launch reads a table of task records and
+// a table of per-workgroup records {task, sub} from one device blob, so a
+// launch can mix matrices of different n and p, and workgroups can be
+// ordered heaviest-first.  All matrices are FP64, column-major.
+#pragma once
+#include <stdint.h>
+
+namespace bsofi {
+
+// Householder QR of one m x nbj sub-panel (columns of one BSOF panel),
+// LAPACK dgeqrf conventions (dlarfg sign rule, unit-lower reflectors stored
+// below the diagonal, tau per column), plus the nbj x nbj upper triangular
+// T of the compact WY form H_1...H_nbj = I - V T V^T.
+struct PanelTask {
+  double* P;      // top-left of the sub-panel in A (its diagonal corner)
+  double* tau;    // tau for the sub-panel's first column
+  double* T;      // T factor out, column-major, leading dimension ldt
+  int* info;      // per-matrix status word (unused by the panel today)
+  int lda, m, nbj, ldt;
+};
+
+// Block reflector application: C <- op(I - V T V^T) C over the m rows of the
+// reflectors, for the column set {C1 .. C1+w1} U {C2 .. C2+w2}.
+// trans = 1 applies (I - V T V^T)^T = I - V T^T V^T (used by BSOF, Q_k^T
+// from the left); trans = 0 applies I - V T V^T (used to build Q_k).
+struct ReflTask {
+  const double* V;  // reflector block top-left (unit diagonal implicit)
+  const double* T;
+  double* C1;
+  double* C2;
+  int ldv, ldt, ldc;
+  int m, nbj, w1, w2, trans;
+};
+
+// Batched tile GEMM: C = alpha*A*B + beta*C (+ D on columns >= dcol0).
+// Triangular operands are read through masks: amask = 1 treats A as upper
+// triangular (A[i][k] = 0 for i > k); bmask = 1 treats B as upper
+// triangular (B[k][j] = 0 for k > j).  Both stay valid for the lower part
+// of the array holding other data (the Householder vectors).
+struct GemmTask {
+  const double* A;
+  const double* B;
+  double* C;
+  const double* D;
+  double alpha, beta;
+  int lda, ldb, ldc, ldd;
+  int M, N, K;
+  int amask, bmask, dcol0, tiles_n;
+};
+
+// Inversion of one diagonal sub-block (<= 32 x 32) of an upper triangular
+// block in place.  gidx0 is the 1-based global column index of the
+// sub-block's first diagonal element, reported through *info (atomicMin) if
+// a diagonal entry is exactly zero.
+struct TrtriTask {
+  double* X;
+  int* info;
+  int ld, nb, gidx0, pad;
+};
+
+// One row tile of the BSOI scan: rows [r0, r0 + rows) of matrix A (N = n*p)
+// computed as G = X Q_{p-2}^T ... Q_0^T with X = R^{-1} read in place and
+// the explicit Q_k (2n x 2n each, stride qstride) from the workspace.
+struct ScanTask {
+  double* A;
+  const double* Q;
+  long long qstride;
+  int lda, n, p, r0, rows, pad;
+};
+
+// Identity fill of count square matrices of order dim, stride apart.
+struct EyeTask {
+  double* Q;
+  long long stride;
+  int dim, count;
+};
+
+// Work-item record: which task a workgroup serves and which piece of it.
+struct Item {
+  int task;
+  int sub;
+};
+
+}  // namespace bsofi

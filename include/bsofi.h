@@ -1,0 +1,228 @@
+/*
+ * bsofi.h -- C ABI of libbsofi_amd.so, an MI355X (gfx950) implementation of
+ * BSOF/I: structured orthogonal factorization and explicit inversion of
+ * block p-cyclic matrices (N = n*L, column-major, FP64, in place).
+ *
+ * Part 1 keeps every entry point of SGo-Go/BSOFI's C and Fortran API with
+ * the same names, argument meaning, A/tau/workspace layout and info codes;
+ * each line cites the reference declaration it replaces.  All of them run
+ * on the GPU; there is no CPU fallback.  When no GPU is usable they return
+ * info = BSOFI_ERR_NO_DEVICE and leave the outputs untouched.
+ *
+ * Part 2 is the device-pointer API (the reference's gpuX* family).
+ * Part 3 is the native API: contexts with their own HIP stream, batched
+ * inversion of many matrices of different n and p in one launch sequence,
+ * reusable plans that can be captured in a HIP graph, the DQMC generator,
+ * and the serving loop.  Part 3 has no counterpart in the reference.
+ *
+ * Argument errors follow LAPACK: info = -i for the i-th argument.
+ * info > 0 reports the 1-based global index of the first exactly-zero
+ * diagonal entry of R (R singular), as dtrtri does for one block.
+ */
+#ifndef BSOFI_AMD_H
+#define BSOFI_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#define BSOFI_ERR_NO_DEVICE (-1000) /* no usable GPU / HIP runtime error */
+#define BSOFI_ERR_ALLOC (-1001)     /* device allocation failed */
+#define BSOFI_ERR_UNSUPPORTED (-1002) /* n beyond the supported range */
+#define BSOFI_ERR_HANDLE (-1003)    /* uninitialised or finalized H-API handle */
+
+/* ======================================================================
+ * Part 1: reference-compatible entry points (host arrays)
+ * ====================================================================== */
+
+/* include/bsofi_cpu.h:24 -- host workspace (doubles) for the calls below. */
+int lapackXbsofiLWork(int n, int L, int lda);
+/* include/bsofi_cpu.h:26-28 -- BSOF: A <- V + R, tau (N). */
+int lapackXbsof(int n, int L, double A[], int lda, double tau[], double work[], int lwork,
+                int* info);
+/* include/bsofi_cpu.h:38-41 -- BSTRI: upper block triangle R <- R^{-1}. */
+int lapackXbstri(int n, int L, double A[], int lda, int* info);
+/* include/bsofi_cpu.h:34-36 -- BSOI: A <- G = R^{-1} Q^T (reflector form). */
+int lapackXbsoi_ormqr(int n, int L, double A[], int lda, double tau[], double work[], int lwork,
+                      int* info);
+/* include/bsofi_cpu.h:30-32 -- BSOI with explicit Q_k (same result). */
+int lapackXbsoi_orgqr(int n, int L, double A[], int lda, double tau[], double work[], int lwork,
+                      int* info);
+/* bsofi.h:36-37 of the reference: the default BSOI is the dormqr one. */
+#define lapackXbsoi lapackXbsoi_ormqr
+
+/* include/bsofi_cpu.h:48-72 / src/bsofi_callback.c:24-56 -- Fortran names. */
+int lapackxbsofilwork_(int* n, int* L, int* lda);
+int lapackxbsof_(int* n, int* L, double A[], int* lda, double tau[], double work[], int* lwork,
+                 int* info);
+int lapackxbstri_(int* n, int* L, double A[], int* lda, int* info);
+int lapackxbsoi_ormqr_(int* n, int* L, double A[], int* lda, double tau[], double work[],
+                       int* lwork, int* info);
+int lapackxbsoi_orgqr_(int* n, int* L, double A[], int* lda, double tau[], double work[],
+                       int* lwork, int* info);
+
+/* include/bsofi_hybrid.h:22 -- lworkHostDevice = -1: host doubles,
+ * -2: device doubles for the dwork argument below. */
+int hybridXbsofiLWork(int n, int L, int lda, int lworkHostDevice);
+/* include/bsofi_hybrid.h:32-37 -- BSOF + BSTRI.  The first argument took a
+ * cuBLAS handle; it is kept as an opaque pointer-sized value and ignored.
+ * dwork (device, ldwork doubles) is used as the whole device workspace when
+ * ldwork >= hybridXbsofiLWork(n, L, lda, -2); otherwise the library
+ * allocates its own.  On exit A = V + R^{-1} and tau as in the reference. */
+int hybridXbsoftri(void* handle, int n, int L, double A[], int lda, double tau[], double work[],
+                   int lwork, double dwork[], int ldwork, int* info);
+/* include/bsofi_hybrid.h:39-43 -- BSOI; A = G on exit. */
+int hybridXbsoi(void* handle, int n, int L, double A[], int lda, double tau[], double work[],
+                int lwork, double dwork[], int ldwork, int* info);
+/* include/bsofi_hybrid.h:39-40 -- aliases kept by the reference. */
+#define hybridXbsof lapackXbsof
+#define hybridXbstri lapackXbstri
+
+/* include/bsofi_hybrid.h:56-74 / src/bsofi_callback.c:62-87 -- Fortran. */
+int hybridxbsofilwork_(int* n, int* L, int* lda, int* lworkHostDevice);
+int hybridxbsoftri_(int* n, int* L, double A[], int* lda, double tau[], double work[], int* lwork,
+                    double dwork[], int* ldwork, int* info);
+int hybridxbsoi_(int* n, int* L, double A[], int* lda, double tau[], double work[], int* lwork,
+                 double dwork[], int* ldwork, int* info);
+
+/* bsofi.h:56-70 / include/bsofi_hybrid.h:77-91 / src/bsofi_callback.c:101-180
+ * -- the simplified handle API.  The handle lives in the caller's integer
+ * array (at least 8 Fortran integers, as the reference's 32-byte struct):
+ * {magic, n, L, lda, context pointer}, 24 bytes.  init allocates a device
+ * context with its own stream and workspace; bsoftriH uploads A, runs BSOF
+ * and BSTRI, writes A = V + R^{-1} and tau back; bsoiH reuses the device
+ * copy when (A, n, L, lda) match the preceding bsoftriH on this handle (set
+ * BSOFI_HAPI_REUPLOAD=1 to always re-upload) and writes A = G back. */
+int hybridxinith_(int* handle, int* n, int* L, int* lda);
+int hybridxbsofilworkh_(int* handle, int* n, int* L, int* lda);
+int hybridxbsoftrih_(int* handle, int* n, int* L, double A[], int* lda, double tau[],
+                     double work[], int* lwork, int* info);
+int hybridxbsoih_(int* handle, int* n, int* L, double A[], int* lda, double tau[], double work[],
+                  int* lwork, int* info);
+int hybridxfinalizeh_(int* handle);
+
+/* src/bsofi_callback.c:189-218 -- wall-clock helpers for Fortran callers. */
+void fgetwalltime_(double tv[2]);
+double felapsed_(double t1[2], double t0[2]);
+
+/* ======================================================================
+ * Part 2: device-pointer API (include/bsofi_gpu.h:34-62)
+ * dA is device memory with leading dimension ldda; tau and work are host
+ * arrays as in the reference.  dT is accepted for signature compatibility
+ * and not used: the BSOI phase rebuilds the T factors from the reflectors in
+ * dA and tau (a -2 query on gpuXbsof returns 1).  dwork may be NULL, or
+ * device memory of (-2 query) doubles used as the whole workspace.
+ * ====================================================================== */
+int gpuXbsof(int n, int L, double dA[], int ldda, double tau[], double work[], int lwork,
+             double* dT, int* info);
+int gpuXbstri_rseq(int n, int L, double dA[], int ldda, double dwork[], int ldwork, int* info);
+int gpuXbstri_rbatched(int n, int L, double dA[], int ldda, double dwork[], int ldwork,
+                       int* info);
+int gpuXbsoi_ormqr(int n, int L, double dA[], int ldda, double tau[], double work[], int lwork,
+                   double* dT, double dwork[], int ldwork, int* info);
+int gpuXbsoi_orgqr(int n, int L, double dA[], int ldda, double tau[], double work[], int lwork,
+                   double* dT, double dwork[], int ldwork, int* info);
+int gpuXbsoi_batched(int n, int L, double dA[], int ldda, double tau[], double work[], int lwork,
+                     double* dT, double dwork[], int ldwork, int* info);
+#define gpuXbstri gpuXbstri_rseq
+#define gpuXbsoi gpuXbsoi_batched
+
+/* ======================================================================
+ * Part 3: native API (no counterpart in the reference)
+ * ====================================================================== */
+typedef struct bsofi_ctx_s* bsofi_ctx_t;
+typedef struct bsofi_plan_s* bsofi_plan_t;
+
+/* One p-cyclic matrix resident in device memory. */
+typedef struct {
+  double* A;   /* device, lda x N, column-major, in place */
+  double* tau; /* device, N doubles */
+  int n, p, lda;
+  int pad;
+} bsofi_matrix_t;
+
+#define BSOFI_PHASE_BSOF 1
+#define BSOFI_PHASE_BSTRI 2
+#define BSOFI_PHASE_BSOI 4
+#define BSOFI_PHASE_ALL 7
+
+/* Contexts.  stream = NULL makes the context create its own stream. */
+int bsofi_device_count(void);
+int bsofi_ctx_create(bsofi_ctx_t* ctx, int device, void* hip_stream);
+int bsofi_ctx_destroy(bsofi_ctx_t ctx);
+int bsofi_ctx_set_stream(bsofi_ctx_t ctx, void* hip_stream);
+void* bsofi_ctx_stream(bsofi_ctx_t ctx);
+int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
+/* Supported block sizes: 1 <= n <= BSOFI_MAX_N, L >= 2. */
+#define BSOFI_MAX_N 512
+int bsofi_check_args(int n, int L, int lda);
+
+/* Device workspace (bytes) one matrix of order (n, p) needs. */
+size_t bsofi_workspace_bytes(int n, int p);
+
+/* Plans: the launch sequence for a batch and a phase set, built once and
+ * executed many times (optionally as a HIP graph).  workspace may be NULL
+ * (the plan allocates it) or device memory of bsofi_plan_workspace_bytes. */
+size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int phases);
+int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases,
+                      void* workspace, bsofi_plan_t* plan);
+int bsofi_plan_execute(bsofi_plan_t plan);
+int bsofi_plan_use_graph(bsofi_plan_t plan, int on);
+int bsofi_plan_num_launches(bsofi_plan_t plan);
+/* Per-matrix status after execution (host copy, count ints). */
+int bsofi_plan_info(bsofi_plan_t plan, int* info);
+/* Time every launch of one kernel family with HIP events on the plan's
+ * stream; read the accumulated milliseconds and launch count after a sync.
+ * family: 0 panel QR, 1 reflector apply, 2 tile GEMM, 3 diag TRTRI,
+ * 4 identity fill, 5 BSOI scan, 6 T factor. */
+int bsofi_plan_time_family(bsofi_plan_t plan, int family);
+int bsofi_plan_family_time(bsofi_plan_t plan, double* ms, int* launches);
+int bsofi_plan_destroy(bsofi_plan_t plan);
+
+/* One-shot batched inversion on device (builds, runs and frees a plan). */
+int bsofi_invert_batched(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases,
+                         int* info);
+
+/* Relative residual ||A G - I||_F / (||A||_F ||G||_F) computed on the
+ * device from the 2p non-zero blocks of the original A (blocks in the
+ * compact layout written by bsofi_gather_blocks) and G. */
+int bsofi_gather_blocks(bsofi_ctx_t ctx, const double* dA, int lda, int n, int p,
+                        double* dblocks);
+int bsofi_pcyclic_residual(bsofi_ctx_t ctx, const double* dblocks, const double* dG, int ldg,
+                           int n, int p, double* resid);
+
+/* C = alpha * op(A) * B + beta * C with the library's FP64 MFMA tile GEMM;
+ * amask / bmask read A / B as upper triangular.  Synchronous; for tests. */
+int bsofi_debug_gemm(bsofi_ctx_t ctx, int M, int N, int K, double alpha, const double* dA, int lda,
+                     const double* dB, int ldb, double beta, double* dC, int ldc, int amask, int bmask);
+
+/* Flop counts of include/bsofi_flops.h:19-30 (out[0..2] = BSOF, BSTRI,
+ * BSOI). */
+void bsofi_flops(int n, int L, double out[3]);
+
+/* DQMC generator.  e^{-dtau K} of the periodic lx x ly lattice (hopping t)
+ * into expk (n x n, column-major, n = lx*ly), by the closed form of the
+ * lattice Fourier modes. */
+int bsofi_dqmc_expk(int lx, int ly, double t, double dtau, double* expk);
+/* HS field s_{l,i} in {+1,-1} (l < p, i < n) from the counter-based hash
+ * keyed (seed, instance, l, i); host and device produce the same values. */
+int bsofi_dqmc_fields(unsigned long long seed, int instance, int p, int n, signed char* fields);
+/* Device: A = p-cyclic DQMC matrix from device expk (n x n) and device
+ * fields (p x n int8), with e^{+-lambda} given. */
+int bsofi_dqmc_build(bsofi_ctx_t ctx, double* dA, int lda, int n, int p, const double* dexpk,
+                     const signed char* dfields, double exp_plus, double exp_minus);
+/* Device: flip count HS fields at flat indices l*n + i. */
+int bsofi_dqmc_flip(bsofi_ctx_t ctx, signed char* dfields, const int* dflips, int count);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSOFI_AMD_H */
