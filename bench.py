@@ -1,0 +1,302 @@
+"""BSOF/I benchmark on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+plain single-matrix BSOF/I of a 16x16-lattice Hubbard-DQMC p-cyclic matrix,
+n = 256, p = 64 (N = 16384, beta = 8), FP64, with A already resident in HBM
+when the timed region starts.  One step = one full inversion
+(BSOF -> BSTRI -> BSOI) of one matrix; each timed step inverts a different
+seeded matrix (K device buffers built by the device generator beforehand).
+
+Printed as one JSON line (rank 0):
+* value       whole-job inversions/s over the K timed steps;
+* p50/p99     per-inversion latency from HIP events on the library stream;
+* roofline    the dominant kernel (BSOI row scan, ~82% of the flops):
+              FLOPS_BSOI(n,p) (include/bsofi_flops.h:27-28) per launch over
+              its average launch duration (HIP events around every scan
+              launch in the timed region), against the FP64 MFMA peak;
+* cpu_baseline the reference's own CPU path (oracle/_ref: src/bsofi_cpu.c
+              built against MKL) on the host cores, one full inversion of the
+              same matrix (instance 0) as the bounded sample;
+* batched     (--batched) the 512 x (n=64, p=32) uniform batch in one plan,
+              captured in a HIP graph, against the plain path on the same
+              matrix size.
+Every GPU result is checked: ||A G - I||_F / (||A||_F ||G||_F) <= 100 N eps,
+computed on the device from the 2p non-zero blocks of A.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "p-cyclic inversions/s at resid≤100Nε (batched vs plain, 1 GPU); p50/p99 ms"
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix and vector)
+EPS = float(np.finfo(np.float64).eps)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class DeviceSet:
+    """K seeded DQMC matrices (and their 2p non-zero blocks) on the device."""
+
+    def __init__(self, B, ctx, lx, p, count, seed, U=4.0, dtau=0.125, t=1.0, first_instance=0):
+        import torch
+
+        from bsofi_amd import dqmc
+
+        self.n, self.p = lx * lx, p
+        self.N = self.n * p
+        self.lda = self.N
+        self.count = count
+        ek = B.dqmc_expk(lx, lx, t, dtau)
+        lam = dqmc.hs_lambda(U, dtau)
+        self.ep, self.em = math.exp(lam), math.exp(-lam)
+        self.d_ek = torch.from_numpy(ek.T.copy()).cuda()  # column-major n x n
+        self.A, self.tau, self.blocks, self.fields = [], [], [], []
+        for k in range(count):
+            f = B.dqmc_fields(seed, first_instance + k, p, self.n)
+            df = torch.from_numpy(f.copy()).cuda()
+            A = torch.empty((self.N, self.lda), dtype=torch.float64, device="cuda")
+            self.A.append(A)
+            self.fields.append(df)
+            self.tau.append(torch.zeros(self.N, dtype=torch.float64, device="cuda"))
+            self.blocks.append(torch.empty(2 * p * self.n * self.n, dtype=torch.float64, device="cuda"))
+        self.B, self.ctx = B, ctx
+        torch.cuda.synchronize()
+        self.rebuild()
+
+    def rebuild(self, idx=None):
+        B = self.B
+        for k in (range(self.count) if idx is None else idx):
+            rc = B.lib.bsofi_dqmc_build(self.ctx.h, self.A[k].data_ptr(), self.lda, self.n, self.p,
+                                        self.d_ek.data_ptr(), self.fields[k].data_ptr(), self.ep, self.em)
+            assert rc == 0, rc
+            B.gather_blocks(self.ctx, self.A[k].data_ptr(), self.lda, self.n, self.p, self.blocks[k].data_ptr())
+        self.ctx.synchronize()
+
+    def desc(self, k):
+        return (self.A[k].data_ptr(), self.tau[k].data_ptr(), self.n, self.p, self.lda)
+
+    def residual(self, k):
+        return self.B.pcyclic_residual(self.ctx, self.blocks[k].data_ptr(), self.A[k].data_ptr(), self.lda, self.n,
+                                       self.p)
+
+
+def bench_plain(B, lx, p, steps, warmup, seed):
+    import torch
+
+    ctx = B.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    nbuf = steps + 1
+    log(f"[bench] building {nbuf} device matrices n={lx * lx} p={p}")
+    ds = DeviceSet(B, ctx, lx, p, nbuf, seed)
+    plans = [B.Plan(ctx, [ds.desc(k)]) for k in range(nbuf)]
+    log(f"[bench] plan: {plans[0].num_launches} launches per inversion")
+    # warm-up on buffer `steps` (rebuilt between warm-up steps, outside timing)
+    for w in range(warmup):
+        plans[steps].execute()
+        ctx.synchronize()
+        ds.rebuild([steps])
+        log(f"[bench] warmup {w + 1}/{warmup} done")
+    for pl in plans[:steps]:
+        pl.time_family("scan")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for k in range(steps):
+            plans[k].execute()
+            ev[k + 1].record(stream)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)])
+    scan_ms, scan_n = 0.0, 0
+    for pl in plans[:steps]:
+        ms, cnt = pl.family_time()
+        scan_ms += ms
+        scan_n += cnt
+    infos = [pl.info()[0] for pl in plans[:steps]]
+    res = [ds.residual(k) for k in range(steps)]
+    for pl in plans:
+        pl.close()
+    out = dict(n=ds.n, p=p, N=ds.N, wall=wall, lat=lat, scan_ms=scan_ms / max(scan_n, 1), scan_launches=scan_n,
+               infos=infos, resid=res)
+    del ds
+    ctx.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_batched(B, lx, p, count, reps, seed):
+    """count matrices in one plan (HIP graph), reps executions (inputs
+    rebuilt between executions, outside the timed interval)."""
+    import torch
+
+    ctx = B.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    log(f"[bench] batched: building {count} matrices n={lx * lx} p={p}")
+    ds = DeviceSet(B, ctx, lx, p, count, seed, first_instance=1000)
+    plan = B.Plan(ctx, [ds.desc(k) for k in range(count)])
+    plan.use_graph(True)
+    plan.execute()  # warm-up + graph instantiation
+    ctx.synchronize()
+    ds.rebuild()
+    times = []
+    for r in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            plan.execute()
+            e1.record(stream)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+        if r + 1 < reps:
+            ds.rebuild()
+    res = max(ds.residual(k) for k in range(0, count, max(1, count // 16)))
+    infos = plan.info()
+    plan.close()
+    del ds
+    ctx.close()
+    torch.cuda.empty_cache()
+    return dict(ms=float(np.median(times)), resid=res, info_bad=sum(1 for i in infos if i != 0))
+
+
+def cpu_baseline(lx, p, seed, threads):
+    """The reference's CPU path (oracle/_ref) on the same matrix (instance 0)."""
+    import oracle
+    from bsofi_amd import dqmc
+
+    if oracle.ref_available():
+        impl = oracle.Reference(threads=threads)
+        kind = "reference"
+    else:
+        impl = oracle.Oracle("auto", threads=threads)
+        kind = "port"
+    A0, meta = dqmc.dqmc_matrix(lx, p, seed=seed, instance=0)
+    n = meta["n"]
+    A = A0.copy(order="F")
+    t0 = time.perf_counter()
+    if kind == "reference":
+        tau, info = impl.invert(n, p, A)
+    else:
+        tau, info, _ = impl.invert(n, p, A)
+    dt = time.perf_counter() - t0
+    return dict(seconds=dt, kind=kind, info=info)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--lx", type=int, default=16, help="lattice side (n = lx^2)")
+    ap.add_argument("--p", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--batched", type=int, default=1, help="also run the 512 x (n=64,p=32) batch")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build targets one GPU; multi-GPU runs are out of scope "
+              "(see the sharding design in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import bsofi_amd as B
+    from bsofi_amd import flops
+
+    if B.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed)
+    n, p, N = r["n"], r["p"], r["N"]
+    value = args.steps / r["wall"]
+    tot_flops = flops.bsofi(n, p)
+    bsoi_flops = flops.bsoi(n, p)
+    scan_s = r["scan_ms"] * 1e-3
+    achieved = bsoi_flops / scan_s / 1e12 if scan_s > 0 else 0.0
+    resid_max = max(r["resid"])
+    bound = 100 * N * EPS
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "inversions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["wall"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Hubbard-DQMC p-cyclic matrices (U=4, dtau=0.125, t=1), one per step",
+        "config": {"workload": f"plain single-matrix BSOF/I, {args.lx}x{args.lx} lattice n={n}, p={p}, N={N}, "
+                               "device-resident A, full inverse G formed on the GPU",
+                   "n": n, "p": p, "N": N, "batch": 1},
+        "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+        "p99_ms": round(float(np.percentile(r["lat"], 99)), 3),
+        "gflops": round(tot_flops / (r["wall"] / args.steps) / 1e9, 1),
+        "frac_fp64_peak": round(tot_flops / (r["wall"] / args.steps) / (FP64_PEAK_TFLOPS * 1e12), 4),
+        "resid_max": resid_max,
+        "resid_bound": bound,
+        "resid_ok": bool(resid_max <= bound and all(i == 0 for i in r["infos"])),
+        "roofline": {
+            "kernel": "bsoi_scan_kernel",
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_launch": bsoi_flops,
+            "avg_launch_ms": round(r["scan_ms"], 4),
+            "launches": r["scan_launches"],
+        },
+    }
+    if args.batched:
+        nb_lx, nb_p, count = 8, 32, 512
+        b = bench_batched(B, nb_lx, nb_p, count, 3, args.seed)
+        pl = bench_plain(B, nb_lx, nb_p, 5, 1, args.seed)
+        plain_ips = 5 / pl["wall"]
+        b_ips = count / (b["ms"] * 1e-3)
+        line["batched"] = {
+            "workload": f"{count} x (n={nb_lx * nb_lx}, p={nb_p}) in one plan, HIP graph",
+            "inversions_per_s": round(b_ips, 1),
+            "ms_per_batch": round(b["ms"], 3),
+            "gflops": round(count * flops.bsofi(nb_lx * nb_lx, nb_p) / (b["ms"] * 1e-3) / 1e9, 1),
+            "plain_same_size_inversions_per_s": round(plain_ips, 1),
+            "speedup_vs_plain": round(b_ips / plain_ips, 2),
+            "resid_max_sampled": b["resid"],
+            "info_nonzero": b["info_bad"],
+        }
+    if args.cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: {threads} threads")
+        c = cpu_baseline(args.lx, args.p, args.seed, threads)
+        line["cpu_baseline"] = {
+            "value": round(1.0 / c["seconds"], 5),
+            "unit": "inversions/s",
+            "cores": threads,
+            "kind": c["kind"],
+            "sample": f"one full inversion of the same n={n}, p={p} matrix (instance 0), MKL {threads} threads",
+            "gflops": round(tot_flops / c["seconds"] / 1e9, 2),
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

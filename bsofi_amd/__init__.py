@@ -50,6 +50,13 @@ class bsofi_matrix_t(C.Structure):
 
 
 def _load():
+    # One HIP runtime per process: torch bundles its own libamdhip64 (same
+    # soname, different file).  Loading torch first makes this library bind
+    # to the runtime already in the process instead of mapping a second one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C bsofi_amd/csrc)")
@@ -98,6 +105,7 @@ def _load():
         "bsofi_dqmc_fields": (i, [C.c_ulonglong, i, i, i, C.POINTER(C.c_byte)]),
         "bsofi_dqmc_build": (i, [_vp, _vp, i, i, i, _vp, _vp, d, d]),
         "bsofi_dqmc_flip": (i, [_vp, _vp, _vp, i]),
+        "bsofi_memcpy": (i, [_vp, _vp, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -340,6 +348,14 @@ def gather_blocks(ctx: Context, A_ptr, lda, n, p, blocks_ptr):
     rc = lib.bsofi_gather_blocks(ctx.h, A_ptr, lda, n, p, blocks_ptr)
     if rc:
         raise RuntimeError(f"bsofi_gather_blocks failed ({rc})")
+
+
+def copy_to_host(dptr, n_doubles):
+    out = np.empty(n_doubles)
+    rc = lib.bsofi_memcpy(out.ctypes.data, dptr, out.nbytes)
+    if rc:
+        raise RuntimeError(f"bsofi_memcpy failed ({rc})")
+    return out
 
 
 def dqmc_expk(lx, ly, t=1.0, dtau=0.125):

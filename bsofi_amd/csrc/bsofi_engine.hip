@@ -584,7 +584,10 @@ int bsofi_ctx_create(bsofi_ctx_t* ctx, int device, void* hip_stream) {
   if (hip_stream) {
     c->stream = (hipStream_t)hip_stream;
   } else {
-    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+    // A blocking stream: ordered against the legacy default stream, so data
+    // that a caller (torch, hipMemcpy) produced there is visible to our
+    // kernels without an explicit synchronisation.  Default priority.
+    if (!hip_ok(hipStreamCreate(&c->stream), "hipStreamCreate")) {
       delete c;
       return BSOFI_ERR_NO_DEVICE;
     }
@@ -906,6 +909,11 @@ int bsofi_dqmc_build(bsofi_ctx_t ctx, double* dA, int lda, int n, int p, const d
   hipLaunchKernelGGL(dqmc_build_kernel, dim3(4096), dim3(256), 0, ctx->stream, dA, lda, n, p, dexpk, dfields, ep,
                      em);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bsofi_memcpy(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
   return 0;
 }
 

@@ -19,7 +19,8 @@
 // MFMA: v_mfma_f64_16x16x4_f64.  Operand layouts (64-lane wave, lane l):
 //   A (16 x 4):  lane l holds A[l % 16][l / 16]
 //   B (4 x 16):  lane l holds B[l / 16][l % 16]
-//   C (16 x 16): lane l holds C[4 * (l / 16) + r][l % 16], r = 0..3
+//   C (16 x 16): accumulator register r of lane l holds C[4 * r + l / 16][l % 16]
+//                (crow() below; checked on the device by the GEMM tests)
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <math.h>
@@ -33,6 +34,10 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+
+// Row, inside a 16 x 16 accumulator tile, of register r of a lane whose
+// lane / 16 is lk.
+__device__ __forceinline__ int crow(int lk, int r) { return 4 * r + lk; }
 
 __device__ __forceinline__ d4 d4zero() {
   d4 z = {0.0, 0.0, 0.0, 0.0};
@@ -335,8 +340,8 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    Wp[wave][4 * lk + r][li] = acc0[r];
-    Wp[wave][16 + 4 * lk + r][li] = two ? acc1[r] : 0.0;
+    Wp[wave][crow(lk, r)][li] = acc0[r];
+    Wp[wave][16 + crow(lk, r)][li] = two ? acc1[r] : 0.0;
   }
   for (int e = tid; e < 32 * 32; e += 256) {
     const int i = e & 31, j = e >> 5;
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
     d4 acc;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = rt * 16 + 4 * lk + r;
+      const int row = rt * 16 + crow(lk, r);
       acc[r] = (cptr && row < m) ? cptr[row] : 0.0;
     }
     for (int k0 = 0; k0 < nbj; k0 += 4) {
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
     if (cptr) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = rt * 16 + 4 * lk + r;
+        const int row = rt * 16 + crow(lk, r);
         if (row < m) cptr[row] = acc[r];
       }
     }
@@ -425,7 +430,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Gp[wave][a * 16 + 4 * lk + r][b * 16 + li] = acc[a][b][r];
+      for (int r = 0; r < 4; ++r) Gp[wave][a * 16 + crow(lk, r)][b * 16 + li] = acc[a][b][r];
   if (tid < 32) taus[tid] = (tid < nbj) ? t.tau[tid] : 0.0;
   __syncthreads();
   for (int e = tid; e < 32 * 32; e += 256) {
@@ -507,7 +512,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmTask* __restrict__ 
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = i0 + wi * 32 + a * 16 + 4 * lk + r;
+        const int row = i0 + wi * 32 + a * 16 + crow(lk, r);
         if (row >= M) continue;
         double v = t.alpha * acc[a][b][r];
         double* cp = t.C + (size_t)col * t.ldc + row;
@@ -684,7 +689,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
       for (int a = 0; a < MTI; ++a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int i = a * 16 + 4 * lk + r;
+          const int i = a * 16 + crow(lk, r);
           if (col < n) {
             carry[col * ldcar + i] = acc[a][b][r];
           } else {
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(256) void pcyclic_residual_kernel(const double* __r
       const int col = j0 + wj * 32 + b * 16 + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = i0 + wi * 32 + a * 16 + 4 * lk + r;
+        const int row = i0 + wi * 32 + a * 16 + crow(lk, r);
         if (row < n && col < N) {
           double v = acc[a][b][r];
           if (ib * n + row == col) v -= 1.0;

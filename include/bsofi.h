@@ -201,6 +201,9 @@ int bsofi_pcyclic_residual(bsofi_ctx_t ctx, const double* dblocks, const double*
 int bsofi_debug_gemm(bsofi_ctx_t ctx, int M, int N, int K, double alpha, const double* dA, int lda,
                      const double* dB, int ldb, double beta, double* dC, int ldc, int amask, int bmask);
 
+/* Synchronous copy between any host/device pointers (hipMemcpyDefault). */
+int bsofi_memcpy(void* dst, const void* src, size_t bytes);
+
 /* Flop counts of include/bsofi_flops.h:19-30 (out[0..2] = BSOF, BSTRI,
  * BSOI). */
 void bsofi_flops(int n, int L, double out[3]);
@@ -218,6 +221,29 @@ int bsofi_dqmc_build(bsofi_ctx_t ctx, double* dA, int lda, int n, int p, const d
                      const signed char* dfields, double exp_plus, double exp_minus);
 /* Device: flip count HS fields at flat indices l*n + i. */
 int bsofi_dqmc_flip(bsofi_ctx_t ctx, signed char* dfields, const int* dflips, int count);
+
+/* Serving loop (continuous batching of DQMC inversion requests).
+ * A chain owns a lattice (lx x ly, hopping t, dtau), p, U and an HS field
+ * seeded by (seed, instance); e^{-dtau K} of each lattice is cached on the
+ * device.  submit() flips fields (flat indices l*n + i) and queues a
+ * request that ships only its field snapshot; step() retires finished
+ * batches, admits queued requests FIFO into free stream slots while their
+ * device memory (G, tau, workspace) fits the HBM budget, and launches one
+ * batched plan per slot.  Results stay on the device until release(). */
+typedef struct bsofi_server_s* bsofi_server_t;
+int bsofi_server_create(int device, size_t hbm_budget_bytes, int nslots, int max_batch, bsofi_server_t* srv);
+int bsofi_server_destroy(bsofi_server_t srv);
+int bsofi_server_add_chain(bsofi_server_t srv, int lx, int ly, int p, double U, double dtau, double t,
+                           unsigned long long seed, int instance, int* chain_id);
+int bsofi_server_submit(bsofi_server_t srv, int chain, const int* flips, int nflips, long long* request_id);
+/* Returns the queue length (>= 0) or an error (< 0). */
+int bsofi_server_step(bsofi_server_t srv, int* inflight);
+int bsofi_server_poll(bsofi_server_t srv, long long* ids, double* latency_s, double* queue_wait_s, int* info,
+                      int max);
+int bsofi_server_result(bsofi_server_t srv, long long id, double** dG, double** dtau, int* n, int* p);
+int bsofi_server_release(bsofi_server_t srv, long long id);
+int bsofi_server_stats(bsofi_server_t srv, long long* completed, int* queued, size_t* used_bytes);
+int bsofi_server_request_bytes(int n, int p, size_t* bytes);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

@@ -23,6 +23,11 @@ import os
 
 import numpy as np
 
+# MKL's default Intel OpenMP threading layer computes wrong results when
+# libgomp is also in the process (torch loads it): use the GNU layer, which
+# shares libgomp.  Must be set before MKL initialises.
+os.environ.setdefault("MKL_THREADING_LAYER", "GNU")
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libbsofi_ref.so")

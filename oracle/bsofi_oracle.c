@@ -359,6 +359,9 @@ int oracle_set_backend(int backend, const char* path) {
   }
   const char* prefix = (backend == BACKEND_OPENBLAS) ? "scipy_" : "";
   if (!path || !path[0]) return -1;
+  /* MKL's default Intel OpenMP layer gives wrong results when libgomp (which
+   * torch loads) is in the same process; the GNU layer shares libgomp. */
+  if (backend == BACKEND_MKL) setenv("MKL_THREADING_LAYER", "GNU", 0);
   g_handle = dlopen(path, RTLD_NOW | RTLD_LOCAL);
   if (!g_handle) return -1;
   g_la.geqrf = (geqrf_t)resolve(prefix, "dgeqrf_");

@@ -1,0 +1,215 @@
+"""GPU: batched plans, HIP-graph replay, and the serving loop.
+
+* a batch of matrices of different n and p in one plan gives, for every
+  matrix, exactly the bits it gets alone (per-matrix tasks, no cross-matrix
+  reductions), and matches the oracle to the stated tolerance;
+* a plan captured in a HIP graph and replayed several times, with eager
+  work and synchronisations in between, reproduces the eager result bit for
+  bit on every replay;
+* the device DQMC generator reproduces the host generator bit for bit;
+* the serving loop returns, for a request served alone and the same request
+  served in a mix of concurrent requests from other chains, the same G.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from bsofi_amd import dqmc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def B():
+    import bsofi_amd
+
+    if bsofi_amd.device_count() < 1:
+        pytest.fail("GPU tests need a GPU")
+    return bsofi_amd
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def to_dev(A):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(A.T)).cuda()  # row-major (N, lda) == column-major A
+
+
+def from_dev(dA, N):
+    return dA.cpu().numpy().T[:N, :N]
+
+
+MIX = [(2, 3), (3, 7), (6, 5), (8, 4), (4, 12), (10, 3), (1, 9)]  # (lattice side, p)
+
+
+def test_batched_mix_equals_alone_and_oracle(B, eps):
+    import torch
+
+    ctx = B.Context(0)
+    inputs, devs, taus = [], [], []
+    for k, (lx, p) in enumerate(MIX):
+        A0, m = dqmc.dqmc_matrix(lx, p, instance=k)
+        inputs.append((A0, m["n"], p))
+        devs.append(to_dev(A0))
+        taus.append(torch.zeros(m["n"] * p, dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+    descs = [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t, (_, n, p) in zip(devs, taus, inputs)]
+    info = B.invert_batched(ctx, descs)
+    assert info == [0] * len(MIX)
+    mixed = [from_dev(d, n * p).copy() for d, (_, n, p) in zip(devs, inputs)]
+    O = oracle.Oracle("auto", threads=1)
+    for k, (A0, n, p) in enumerate(inputs):
+        print(f"batched case {k}: n={n} p={p}", flush=True)
+        d = to_dev(A0)
+        t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+        alone = from_dev(d, n * p)
+        assert np.array_equal(alone, mixed[k]), f"case {k} differs between alone and mix"
+        G = A0.copy(order="F")
+        O.invert(n, p, G)
+        assert rel(alone, G) <= 1e-10
+        assert oracle.residual(A0, alone) <= 100 * n * p * eps
+    ctx.close()
+
+
+def test_graph_replays_match_eager(B):
+    import torch
+
+    ctx = B.Context(0)
+    cases = [(4, 6), (8, 5), (3, 9)]
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=20 + k)[0] for k, (lx, p) in enumerate(cases)]
+    dims = [(lx * lx, p) for lx, p in cases]
+    devs = [to_dev(A) for A in inputs]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in dims]
+    torch.cuda.synchronize()
+    descs = [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t, (n, p) in zip(devs, taus, dims)]
+    eager = B.Plan(ctx, descs)
+    eager.execute()
+    ctx.synchronize()
+    ref = [from_dev(d, n * p).copy() for d, (n, p) in zip(devs, dims)]
+    graph = B.Plan(ctx, descs)
+    graph.use_graph(True)
+    for rep in range(4):
+        for d, A in zip(devs, inputs):
+            d.copy_(to_dev(A))
+        torch.cuda.synchronize()
+        graph.execute()
+        ctx.synchronize()
+        # eager work between replays
+        x = torch.randn(256, 256, dtype=torch.float64, device="cuda")
+        (x @ x).sum().item()
+        for k, (d, (n, p)) in enumerate(zip(devs, dims)):
+            assert np.array_equal(from_dev(d, n * p), ref[k]), f"replay {rep}, matrix {k}"
+        assert graph.info() == [0, 0, 0]
+    eager.close()
+    graph.close()
+    ctx.close()
+
+
+def test_device_generator_matches_host(B):
+    import torch
+
+    ctx = B.Context(0)
+    for lx, p, inst in [(2, 3, 0), (8, 16, 5), (6, 10, 9)]:
+        A0, m = dqmc.dqmc_matrix(lx, p, instance=inst)
+        n, N = m["n"], m["n"] * p
+        ek = B.dqmc_expk(lx, lx)
+        dek = torch.from_numpy(m["expk"].T.copy()).cuda()  # same expk as the host twin
+        f = B.dqmc_fields(dqmc.DEFAULT_SEED, inst, p, n)
+        df = torch.from_numpy(f.copy()).cuda()
+        dA = torch.full((N, N), 3.0, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        lam = m["lam"]
+        assert B.lib.bsofi_dqmc_build(ctx.h, dA.data_ptr(), N, n, p, dek.data_ptr(), df.data_ptr(), math.exp(lam),
+                                      math.exp(-lam)) == 0
+        ctx.synchronize()
+        assert np.array_equal(from_dev(dA, N), A0)
+        assert np.allclose(ek, m["expk"], atol=1e-14)
+    ctx.close()
+
+
+def test_device_residual_matches_host(B):
+    import torch
+
+    ctx = B.Context(0)
+    lx, p = 4, 6
+    A0, m = dqmc.dqmc_matrix(lx, p)
+    n, N = m["n"], m["n"] * p
+    dA = to_dev(A0)
+    blocks = torch.empty(2 * p * n * n, dtype=torch.float64, device="cuda")
+    tau = torch.zeros(N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    B.gather_blocks(ctx, dA.data_ptr(), N, n, p, blocks.data_ptr())
+    assert B.invert_batched(ctx, [(dA.data_ptr(), tau.data_ptr(), n, p, N)]) == [0]
+    G = from_dev(dA, N)
+    r_dev = B.pcyclic_residual(ctx, blocks.data_ptr(), dA.data_ptr(), N, n, p)
+    r_host = oracle.residual(A0, G)
+    assert abs(r_dev - r_host) <= 0.5 * r_host + 1e-18
+    G2 = G.copy()
+    G2[5, 7] += 1e-7
+    dA.copy_(to_dev(G2))
+    torch.cuda.synchronize()
+    r_dev2 = B.pcyclic_residual(ctx, blocks.data_ptr(), dA.data_ptr(), N, n, p)
+    assert abs(r_dev2 - oracle.residual(A0, G2)) <= 1e-3 * r_dev2
+    ctx.close()
+
+
+def test_serving_alone_equals_mix(B, eps):
+    import torch
+
+    from bsofi_amd import serve
+
+    def grab(srv, rid):
+        g, t, n, p = srv.result(rid)
+        N = n * p
+        return B.copy_to_host(g, N * N).reshape(N, N).T.copy()  # column-major N x N
+
+    flips = np.array([3, 17, 40, 41, 99], dtype=np.int32)
+    # alone
+    s1 = serve.Server(budget_bytes=8 << 30, slots=2, max_batch=8)
+    c = s1.add_chain(4, 8, instance=3)
+    rid = s1.submit(c, flips)
+    got = None
+    for _ in range(100000):
+        s1.step()
+        done = s1.poll()
+        if done:
+            assert done[0][3] == 0
+            got = grab(s1, rid)
+            s1.release(rid)
+            break
+    assert got is not None
+    s1.close()
+    # in a mix: other chains (different n, p) submitted before and after
+    s2 = serve.Server(budget_bytes=8 << 30, slots=3, max_batch=4)
+    others = [s2.add_chain(lx, p, instance=50 + k) for k, (lx, p) in enumerate([(6, 5), (3, 11), (8, 4)])]
+    c2 = s2.add_chain(4, 8, instance=3)
+    ids = [s2.submit(o, np.array([1, 2], dtype=np.int32)) for o in others]
+    rid2 = s2.submit(c2, flips)
+    ids += [s2.submit(o, np.array([5], dtype=np.int32)) for o in others]
+    results = {}
+    for _ in range(200000):
+        s2.step()
+        for r, lat, qw, info in s2.poll():
+            assert info == 0
+            if r == rid2:
+                results[r] = grab(s2, r)
+            s2.release(r)
+            results.setdefault(r, None)
+        if len(results) == len(ids) + 1:
+            break
+    assert rid2 in results and results[rid2] is not None
+    assert np.array_equal(results[rid2], got)
+    s2.close()
+    # and it is the inverse of the chain's matrix after the flips
+    A0, m = dqmc.dqmc_matrix(4, 8, instance=3)
+    s = m["fields"].ravel().copy()
+    s[flips] *= -1
+    A = dqmc.assemble(m["expk"], s.reshape(8, 16), m["lam"], 8)
+    assert oracle.residual(A, got) <= 100 * 128 * eps
