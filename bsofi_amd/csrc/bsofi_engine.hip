@@ -183,6 +183,14 @@ struct Planner {
     out.push_back(L);
   }
 
+  // refl_apply_kernel instance: row tiles per wave for the largest m
+  static int refl_variant(const std::vector<ReflTask>& rt) {
+    int mm = 0;
+    for (auto& q : rt) mm = std::max(mm, q.m);
+    const int tiles = (mm + 63) / 64;
+    return tiles <= 1 ? 1 : tiles <= 2 ? 2 : tiles <= 4 ? 4 : tiles <= 8 ? 8 : 16;
+  }
+
   // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------
   void bsof(std::vector<MatJob>& mats) {
     struct Step { int k, j; };
@@ -239,7 +247,7 @@ struct Planner {
         }
       }
       for (int v = 0; v < 5; ++v) emit(F_PANEL, v, pt[v], pi[v]);
-      emit(F_REFL, 0, rt, ri);
+      emit(F_REFL, refl_variant(rt), rt, ri);
     }
   }
 
@@ -428,7 +436,7 @@ struct Planner {
           rt.push_back(q);
         }
       }
-      emit(F_REFL, 0, rt, ri);
+      emit(F_REFL, refl_variant(rt), rt, ri);
     }
     // row scans, grouped by kernel instance; heaviest tiles first
     struct Var { int mti, ntw; };
@@ -518,10 +526,18 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       }
       break;
     }
-    case F_REFL:
-      hipLaunchKernelGGL(refl_apply_kernel, dim3(L.grid), dim3(256), 0, s, (const ReflTask*)(base + L.task_off),
-                         (const Item*)(base + L.item_off));
+    case F_REFL: {
+      auto t = (const ReflTask*)(base + L.task_off);
+      auto it = (const Item*)(base + L.item_off);
+      switch (L.variant) {
+        case 1: hipLaunchKernelGGL(refl_apply_kernel<1>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 2: hipLaunchKernelGGL(refl_apply_kernel<2>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 4: hipLaunchKernelGGL(refl_apply_kernel<4>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 8: hipLaunchKernelGGL(refl_apply_kernel<8>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+        default: hipLaunchKernelGGL(refl_apply_kernel<16>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+      }
       break;
+    }
     case F_GEMM:
       hipLaunchKernelGGL(gemm_kernel, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
                          (const Item*)(base + L.item_off));

@@ -56,32 +56,36 @@ __device__ __forceinline__ d4 d4zero() {
 //   j >  c : the dot products of H_c's application
 //   j <  c : v_j^T v_c        (T factor, dlarft)
 // so each column costs one block-wide reduction and one barrier.
-template <int CPG>
-__device__ __forceinline__ double wave_transpose_reduce(double (&v)[CPG], int lane, int& idx) {
-  // Reduce v[0..CPG) over the 16 lanes sharing lane & 3 (xor 4, 8, 16, 32).
-  // On return the lane holds the wave sum of entry idx.
-  int cnt = CPG;
-  idx = 0;
+// Reduce v[0..CPG) over the 16 lanes sharing lane & 3 (xor 32, 16, 8, 4).
+// Each stage halves the entries a lane keeps ("transpose reduce"), so CPG
+// values cost CPG - 1 + (stages left) shuffles instead of 4 * CPG.  On
+// return the lane holds the wave sum of entry idx.  Stage sizes are
+// template arguments so every array index is a compile-time constant.
+template <int CNT, int MASK, int CPG>
+__device__ __forceinline__ void transpose_stage(double (&v)[CPG], int lane, int& idx) {
+  if constexpr (MASK >= 4) {
+    if constexpr (CNT >= 2) {
+      constexpr int half = CNT / 2;
+      const bool bit = (lane & MASK) != 0;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int mask = 32 >> s;
-    if (cnt >= 2) {
-      const int half = cnt >> 1;
-      const bool bit = (lane & mask) != 0;
-#pragma unroll
-      for (int t = 0; t < CPG / 2; ++t) {
-        if (t < half) {
-          const double send = bit ? v[t] : v[t + half];
-          const double keep = bit ? v[t + half] : v[t];
-          v[t] = keep + __shfl_xor(send, mask);
-        }
+      for (int t = 0; t < half; ++t) {
+        const double send = bit ? v[t] : v[t + half];
+        const double keep = bit ? v[t + half] : v[t];
+        v[t] = keep + __shfl_xor(send, MASK);
       }
       if (bit) idx += half;
-      cnt = half;
+      transpose_stage<half, MASK / 2, CPG>(v, lane, idx);
     } else {
-      v[0] += __shfl_xor(v[0], mask);
+      v[0] += __shfl_xor(v[0], MASK);
+      transpose_stage<CNT, MASK / 2, CPG>(v, lane, idx);
     }
   }
+}
+
+template <int CPG>
+__device__ __forceinline__ double wave_transpose_reduce(double (&v)[CPG], int lane, int& idx) {
+  idx = 0;
+  transpose_stage<CPG, 32, CPG>(v, lane, idx);
   return v[0];
 }
 
@@ -110,23 +114,100 @@ __device__ double block_max_256(double v, double* scratch) {
 // T of the compact WY form (dlarft, forward, columnwise) from the Gram
 // entries G[j][c] = v_j^T v_c (j < c) and tau; lane i owns row i of T:
 //   T[c][c] = tau_c,  T[i][c] = -tau_c * sum_{j=i}^{c-1} T[i][j] G[j][c].
+// Row i is kept in registers (the loops are unrolled), so the only memory
+// traffic is the broadcast read of G.
 template <int NB>
-__device__ void build_T(const double (*G)[NB + 1], double (*Ts)[NB + 1], const double* taus, int nbj,
-                        double* __restrict__ Tout, int ldt, int lane) {
+__device__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj, double* __restrict__ Tout, int ldt,
+                        int lane) {
   if (lane >= NB) return;
   const int i = lane;
+  double Trow[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) Trow[j] = 0.0;
   for (int c = 0; c < nbj; ++c) {
-    double v = 0.0;
-    if (i < c) {
-      double s = 0.0;
-      for (int j = i; j < c; ++j) s = fma(Ts[i][j], G[j][c], s);
-      v = -taus[c] * s;
-    } else if (i == c) {
-      v = taus[c];
-    }
-    Ts[i][c] = v;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < c) s = fma(Trow[j], G[j][c], s);  // Trow[j] = 0 for j < i
+    const double v = (i < c) ? -taus[c] * s : (i == c ? taus[c] : 0.0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j == c) Trow[j] = v;
   }
-  for (int c = 0; c < nbj; ++c) Tout[(size_t)c * ldt + i] = (i < nbj) ? Ts[i][c] : 0.0;
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    if (c < nbj) Tout[(size_t)c * ldt + i] = (i < nbj) ? Trow[c] : 0.0;
+}
+
+// Broadcast lane (lane & ~3) | cc of each quad to the whole quad (DPP).
+template <int CC>
+__device__ __forceinline__ double quad_bcast_c(double v) {
+  constexpr int ctrl = CC | (CC << 2) | (CC << 4) | (CC << 6);
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double quad_bcast(double v, int cc) {
+  switch (cc) {
+    case 0: return quad_bcast_c<0>(v);
+    case 1: return quad_bcast_c<1>(v);
+    case 2: return quad_bcast_c<2>(v);
+    default: return quad_bcast_c<3>(v);
+  }
+}
+
+// ||x|| over the block with scaling by max|x| (two block reductions); the
+// rare path of the panel's dlarfg, kept out of line so the unrolled column
+// loop stays small enough for the instruction cache.
+template <int RPG>
+__device__ __noinline__ double scaled_norm_256(const double* x, double* scratch) {
+  double lm = 0.0;
+  for (int q = 0; q < RPG; ++q) lm = fmax(lm, fabs(x[q]));
+  const double amax = block_max_256(lm, scratch);
+  double ls = 0.0;
+  if (amax > 0.0) {
+    for (int q = 0; q < RPG; ++q) {
+      const double y = x[q] / amax;
+      ls = fma(y, y, ls);
+    }
+  }
+  const double s2 = block_sum_256(ls, scratch);
+  return amax > 0.0 ? amax * sqrt(s2) : 0.0;
+}
+
+// dlarfg from alpha and ||x||: beta, tau and the scale that turns x into
+// the stored reflector, with LAPACK's safe-minimum rescaling loop.
+__device__ __noinline__ void dlarfg_scalar(double alpha, double xnorm, double* tau_o, double* beta_o,
+                                           double* scale_o) {
+  const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+  double tau, beta, scale;
+  if (xnorm == 0.0) {
+    tau = 0.0;
+    beta = alpha;
+    scale = 1.0;
+  } else {
+    beta = -copysign(hypot(alpha, xnorm), alpha);
+    double al = alpha, sc_acc = 1.0;
+    int knt = 0;
+    if (fabs(beta) < safmin) {
+      const double rsafmn = 1.0 / safmin;
+      do {
+        knt++;
+        sc_acc *= rsafmn;
+        beta *= rsafmn;
+        al *= rsafmn;
+      } while (fabs(beta) < safmin && knt < 20);
+      beta = -copysign(hypot(al, xnorm * sc_acc), al);
+    }
+    tau = (beta - al) / beta;
+    scale = sc_acc / (al - beta);
+    for (int q2 = 0; q2 < knt; ++q2) beta *= safmin;
+  }
+  *tau_o = tau;
+  *beta_o = beta;
+  *scale_o = scale;
 }
 
 template <int RPG, int CPG>
@@ -137,13 +218,12 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
   const PanelTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cg = lane & 3;
-  const int rg = (lane >> 2) + 16 * wave;
+  const int rg = (lane >> 2) + 16 * wave;  // rows rg + 64 q; only q = 0 rows can lie at or above a pivot (c < 32)
   const int m = t.m, nbj = t.nbj, lda = t.lda;
 
-  __shared__ double red[2][4][NB];
+  __shared__ double red[2][NB][4];  // [buffer][column][wave] partial sums
   __shared__ double prow[2][NB];
   __shared__ double G[NB][NB + 1];
-  __shared__ double Ts[NB][NB + 1];
   __shared__ double taus[NB];
   __shared__ double scratch[4];
 
@@ -158,116 +238,67 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
     }
   }
 
-  const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-  const int src_base = lane & ~3;
-
   for (int cc = 0; cc < 4; ++cc) {
 #pragma unroll
     for (int jc = 0; jc < CPG; ++jc) {
       const int c = cc * CPG + jc;
       if (c < nbj) {
         const int buf = c & 1;
-        // x = column c at my rows, from the lane (rg, cc) that owns it.
+        // x = column c below the pivot at my rows (zero at and above it)
         double x[RPG];
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) x[q] = __shfl(a[q][jc], src_base | cc);
+        for (int q = 0; q < RPG; ++q) x[q] = __shfl(a[q][jc], (lane & ~3) | cc);
+        x[0] = (rg > c) ? x[0] : 0.0;
         double P[CPG];
 #pragma unroll
-        for (int jj = 0; jj < CPG; ++jj) P[jj] = 0.0;
+        for (int jj = 0; jj < CPG; ++jj) {
+          double acc = x[0] * a[0][jj];
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) {
-          if (rg + 64 * q > c) {
-#pragma unroll
-            for (int jj = 0; jj < CPG; ++jj) P[jj] = fma(x[q], a[q][jj], P[jj]);
-          }
+          for (int q = 1; q < RPG; ++q) acc = fma(x[q], a[q][jj], acc);
+          P[jj] = acc;
         }
         int idx;
         const double ws = wave_transpose_reduce<CPG>(P, lane, idx);
-        if ((lane & 4) == 0) red[buf][wave][cg * CPG + idx] = ws;
+        if ((lane & 4) == 0) red[buf][cg * CPG + idx][wave] = ws;
         if (rg == c) {
 #pragma unroll
           for (int jj = 0; jj < CPG; ++jj) prow[buf][cg * CPG + jj] = a[0][jj];
         }
         __syncthreads();
 
-        double tot[CPG], pr[CPG];
-#pragma unroll
-        for (int jj = 0; jj < CPG; ++jj) {
-          const int j = cg * CPG + jj;
-          tot[jj] = (red[buf][0][j] + red[buf][1][j]) + (red[buf][2][j] + red[buf][3][j]);
-          pr[jj] = prow[buf][j];
-        }
-        double xn2 = (red[buf][0][c] + red[buf][1][c]) + (red[buf][2][c] + red[buf][3][c]);
+        const double xn2 = (red[buf][c][0] + red[buf][c][1]) + (red[buf][c][2] + red[buf][c][3]);
         const double alpha = prow[buf][c];
 
         // dlarfg.  The fast path takes ||x|| = sqrt(sum x^2); a sum of
         // squares outside [2^-960, 2^960] is recomputed with scaling.
-        double xnorm;
-        if (!(xn2 > 0x1p-960) || !(xn2 < 0x1p960)) {
-          double lm = 0.0;
-#pragma unroll
-          for (int q = 0; q < RPG; ++q)
-            if (rg + 64 * q > c) lm = fmax(lm, fabs(x[q]));
-          const double amax = block_max_256(lm, scratch);
-          double ls = 0.0;
-          if (amax > 0.0) {
-#pragma unroll
-            for (int q = 0; q < RPG; ++q)
-              if (rg + 64 * q > c) {
-                const double y = x[q] / amax;
-                ls = fma(y, y, ls);
-              }
-          }
-          const double s2 = block_sum_256(ls, scratch);
-          xnorm = amax > 0.0 ? amax * sqrt(s2) : 0.0;
-        } else {
-          xnorm = sqrt(xn2);
-        }
+        const double xnorm = (!(xn2 > 0x1p-960) || !(xn2 < 0x1p960)) ? scaled_norm_256<RPG>(x, scratch) : sqrt(xn2);
         double tau, beta, scale;
-        if (xnorm == 0.0) {
-          tau = 0.0;
-          beta = alpha;
-          scale = 1.0;
-        } else {
-          beta = -copysign(hypot(alpha, xnorm), alpha);
-          double al = alpha, sc_acc = 1.0;
-          int knt = 0;
-          if (fabs(beta) < safmin) {
-            const double rsafmn = 1.0 / safmin;
-            do {
-              knt++;
-              sc_acc *= rsafmn;
-              beta *= rsafmn;
-              al *= rsafmn;
-            } while (fabs(beta) < safmin && knt < 20);
-            beta = -copysign(hypot(al, xnorm * sc_acc), al);
-          }
-          tau = (beta - al) / beta;
-          scale = sc_acc / (al - beta);
-          for (int q2 = 0; q2 < knt; ++q2) beta *= safmin;
-        }
+        dlarfg_scalar(alpha, xnorm, &tau, &beta, &scale);
 
+        // w_j = tau v_c^T a_j (j > c); G[j][c] = v_j^T v_c (j < c)
         double w[CPG];
 #pragma unroll
         for (int jj = 0; jj < CPG; ++jj) {
           const int j = cg * CPG + jj;
-          const double d = fma(scale, tot[jj], pr[jj]);
+          const double tot = (red[buf][j][0] + red[buf][j][1]) + (red[buf][j][2] + red[buf][j][3]);
+          const double d = fma(scale, tot, prow[buf][j]);
           w[jj] = (j > c) ? tau * d : 0.0;
           if (j < c && rg == 0) G[j][c] = d;
         }
+        // a <- (I - tau v v^T) a with v = 0 above the pivot, 1 at it, x*scale below
+        const double v0 = (rg > c) ? x[0] * scale : (rg == c ? 1.0 : 0.0);
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) {
-          const int row = rg + 64 * q;
-          if (row > c) {
-            const double v = x[q] * scale;
+        for (int jj = 0; jj < CPG; ++jj) a[0][jj] = fma(-v0, w[jj], a[0][jj]);
 #pragma unroll
-            for (int jj = 0; jj < CPG; ++jj) a[q][jj] = fma(-v, w[jj], a[q][jj]);
-            if (cg == cc) a[q][jc] = v;
-          } else if (row == c) {
+        for (int q = 1; q < RPG; ++q) {
+          const double vq = x[q] * scale;
 #pragma unroll
-            for (int jj = 0; jj < CPG; ++jj) a[q][jj] -= w[jj];
-            if (cg == cc) a[q][jc] = beta;
-          }
+          for (int jj = 0; jj < CPG; ++jj) a[q][jj] = fma(-vq, w[jj], a[q][jj]);
+        }
+        if (cg == cc) {
+          a[0][jc] = (rg > c) ? v0 : (rg == c ? beta : a[0][jc]);
+#pragma unroll
+          for (int q = 1; q < RPG; ++q) a[q][jc] = x[q] * scale;
         }
         if (tid == 0) {
           t.tau[c] = tau;
@@ -287,7 +318,7 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
     }
   }
   __syncthreads();
-  if (wave == 0) build_T<NB>(G, Ts, taus, nbj, t.T, t.ldt, lane);
+  if (wave == 0) build_T<NB>(G, taus, nbj, t.T, t.ldt, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,6 +334,12 @@ __device__ __forceinline__ double vload(const double* __restrict__ V, int ldv, i
   return V[(size_t)col * ldv + row];
 }
 
+// The target slab lives in MFMA accumulator registers for the whole kernel:
+// wave w owns row tiles w, w+4, ... (RT of them, m <= 64 RT).  The C layout
+// of a 16 x 16 accumulator tile is exactly four consecutive B fragments of
+// a K = 16 block (register r = rows 4r..4r+3), so phase 1 multiplies
+// straight out of the accumulators and only V is read from memory.
+template <int RT>
 __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
                                                          const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
@@ -312,7 +349,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   const int m = t.m, nbj = t.nbj;
 
   __shared__ double Wp[4][32][17];
-  __shared__ double W[32][17];
+  __shared__ double Wf[32][17];
   __shared__ double Tl[32][33];
 
   // the column this lane serves in B / C fragments
@@ -323,69 +360,87 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   else if (vcol - t.w1 < t.w2)
     cptr = t.C2 + (size_t)(vcol - t.w1) * t.ldc;
 
-  // phase 1
-  const int per = ((m + 15) / 16) * 4;
-  const int kb = wave * per, ke = min(m, kb + per);
-  d4 acc0 = d4zero(), acc1 = d4zero();
-  const bool two = nbj > 16;
-  for (int k0 = kb; k0 < ke; k0 += 4) {
-    const int row = k0 + lk;
-    const double bv = (cptr && row < ke) ? cptr[row] : 0.0;
-    const double a0 = (row < ke) ? vload(t.V, t.ldv, m, nbj, row, li) : 0.0;
-    acc0 = mfma(a0, bv, acc0);
-    if (two) {
-      const double a1 = (row < ke) ? vload(t.V, t.ldv, m, nbj, row, 16 + li) : 0.0;
-      acc1 = mfma(a1, bv, acc1);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    Wp[wave][crow(lk, r)][li] = acc0[r];
-    Wp[wave][16 + crow(lk, r)][li] = two ? acc1[r] : 0.0;
-  }
   for (int e = tid; e < 32 * 32; e += 256) {
     const int i = e & 31, j = e >> 5;
     Tl[i][j] = (i < nbj && j < nbj) ? t.T[(size_t)j * t.ldt + i] : 0.0;
   }
+  d4 acc[RT];
+#pragma unroll
+  for (int s = 0; s < RT; ++s) {
+    const int rt = wave + 4 * s;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rt * 16 + crow(lk, r);
+      acc[s][r] = (cptr && row < m) ? cptr[row] : 0.0;
+    }
+  }
+  // phase 1: W = V^T C
+  const bool two = nbj > 16;
+  d4 w0 = d4zero(), w1 = d4zero();
+#pragma unroll
+  for (int s = 0; s < RT; ++s) {
+    const int rt = wave + 4 * s;
+    if (rt * 16 < m) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + 4 * r + lk;
+        const double a0 = vload(t.V, t.ldv, m, nbj, row, li);
+        const double a1 = two ? vload(t.V, t.ldv, m, nbj, row, 16 + li) : 0.0;
+        w0 = mfma(a0, acc[s][r], w0);
+        if (two) w1 = mfma(a1, acc[s][r], w1);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Wp[wave][crow(lk, r)][li] = w0[r];
+    Wp[wave][16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+  }
   __syncthreads();
+  // phase 2: Wf = -op(T) W
+  __shared__ double W[32][17];
   for (int e = tid; e < 32 * 16; e += 256) {
     const int i = e >> 4, j = e & 15;
     W[i][j] = (Wp[0][i][j] + Wp[1][i][j]) + (Wp[2][i][j] + Wp[3][i][j]);
   }
   __syncthreads();
-  // phase 2: Wp[0] <- -op(T) W
   for (int e = tid; e < 32 * 16; e += 256) {
     const int i = e >> 4, j = e & 15;
-    double s = 0.0;
+    double s2 = 0.0;
     if (i < nbj) {
       if (t.trans) {  // T^T: sum_{k <= i} T[k][i] W[k][j]
-        for (int k = 0; k <= i; ++k) s = fma(Tl[k][i], W[k][j], s);
+#pragma unroll 4
+        for (int k = 0; k <= i; ++k) s2 = fma(Tl[k][i], W[k][j], s2);
       } else {        // T: sum_{k >= i} T[i][k] W[k][j]
-        for (int k = i; k < nbj; ++k) s = fma(Tl[i][k], W[k][j], s);
+#pragma unroll 4
+        for (int k = i; k < nbj; ++k) s2 = fma(Tl[i][k], W[k][j], s2);
       }
     }
-    Wp[0][i][j] = -s;
+    Wf[i][j] = -s2;
   }
   __syncthreads();
-  // phase 3
-  const int mt = (m + 15) / 16;
-  for (int rt = wave; rt < mt; rt += 4) {
-    d4 acc;
+  // phase 3: C += V Wf
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rt * 16 + crow(lk, r);
-      acc[r] = (cptr && row < m) ? cptr[row] : 0.0;
+  for (int s = 0; s < RT; ++s) {
+    const int rt = wave + 4 * s;
+    if (rt * 16 < m) {
+#pragma unroll
+      for (int k0 = 0; k0 < 32; k0 += 4) {
+        if (k0 < nbj) {
+          const double av = vload(t.V, t.ldv, m, nbj, rt * 16 + li, k0 + lk);
+          acc[s] = mfma(av, Wf[k0 + lk][li], acc[s]);
+        }
+      }
     }
-    for (int k0 = 0; k0 < nbj; k0 += 4) {
-      const double av = vload(t.V, t.ldv, m, nbj, rt * 16 + li, k0 + lk);
-      const double bv = Wp[0][k0 + lk][li];
-      acc = mfma(av, bv, acc);
-    }
-    if (cptr) {
+  }
+  if (cptr) {
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      const int rt = wave + 4 * s;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = rt * 16 + crow(lk, r);
-        if (row < m) cptr[row] = acc[r];
+        if (row < m) cptr[row] = acc[s][r];
       }
     }
   }
@@ -406,7 +461,6 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
   const int m = t.m, nbj = t.nbj;
   __shared__ double Gp[4][32][33];
   __shared__ double G[32][33];
-  __shared__ double Ts[32][33];
   __shared__ double taus[32];
   const int per = ((m + 15) / 16) * 4;
   const int kb = wave * per, ke = min(m, kb + per);
@@ -438,7 +492,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
     G[i][j] = (Gp[0][i][j] + Gp[1][i][j]) + (Gp[2][i][j] + Gp[3][i][j]);
   }
   __syncthreads();
-  if (wave == 0) build_T<32>(G, Ts, taus, nbj, t.T, t.ldt, lane);
+  if (wave == 0) build_T<32>(G, taus, nbj, t.T, t.ldt, lane);
 }
 
 __global__ void fill_int_kernel(int* __restrict__ p, int count, int value) {
@@ -629,6 +683,9 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   __syncthreads();
 
   const int rbmin = R0 / n;
+  int bcol[NTW];
+#pragma unroll
+  for (int b = 0; b < NTW; ++b) bcol[b] = (wave + 4 * b) * 16 + li;
   for (int k = p - 2; k >= 0; --k) {
     const double* __restrict__ Qk = t.Q + (size_t)k * t.qstride;
     d4 acc[MTI][NTW];
@@ -637,45 +694,64 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
 #pragma unroll
       for (int b = 0; b < NTW; ++b) acc[a][b] = d4zero();
 
-    int bcol[NTW];
-#pragma unroll
-    for (int b = 0; b < NTW; ++b) bcol[b] = (wave + 4 * b) * 16 + li;
-
+    // Fragments of the next k-step are loaded while the MFMAs of the
+    // current one run (two register buffers, ping-pong).
+    double a0[MTI], b0[NTW], a1[MTI], b1[NTW];
     if (rbmin <= k) {
       // [X_{r,k}] part: K rows 0..n-1 of Q_k^T
-      const size_t colbase = (size_t)k * n * lda;
-      for (int k0 = 0; k0 < n; k0 += 4) {
+      const double* __restrict__ Xk = A + (size_t)k * n * lda;
+      auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
         const int kk = k0 + lk;
-        double af[MTI], bf[NTW];
 #pragma unroll
         for (int a = 0; a < MTI; ++a) {
           double v = 0.0;
-          if (kk < n && rblk[a] <= k && !(rblk[a] == k && rin[a] > kk))
-            v = A[colbase + (size_t)kk * lda + rrow[a]];
+          if (kk < n && rblk[a] <= k && !(rblk[a] == k && rin[a] > kk)) v = Xk[(size_t)kk * lda + rrow[a]];
           af[a] = v;
         }
 #pragma unroll
-        for (int b = 0; b < NTW; ++b)
-          bf[b] = (kk < n && bcol[b] < n2) ? Qk[(size_t)kk * n2 + bcol[b]] : 0.0;
+        for (int b = 0; b < NTW; ++b) bf[b] = (kk < n && bcol[b] < n2) ? Qk[(size_t)kk * n2 + bcol[b]] : 0.0;
+      };
+      load(a0, b0, 0);
+      for (int k0 = 0; k0 < n; k0 += 8) {
+        if (k0 + 4 < n) load(a1, b1, k0 + 4);
 #pragma unroll
         for (int a = 0; a < MTI; ++a)
 #pragma unroll
-          for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(af[a], bf[b], acc[a][b]);
+          for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
+        if (k0 + 8 < n) load(a0, b0, k0 + 8);
+        if (k0 + 4 < n) {
+#pragma unroll
+          for (int a = 0; a < MTI; ++a)
+#pragma unroll
+            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a1[a], b1[b], acc[a][b]);
+        }
       }
     }
     // [carry] part: K rows n..2n-1 of Q_k^T
-    for (int k0 = 0; k0 < n; k0 += 4) {
-      const int kk = k0 + lk;
-      double af[MTI], bf[NTW];
+    {
+      const double* __restrict__ Qc = Qk + (size_t)n * n2;
+      auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
+        const int kk = k0 + lk;
 #pragma unroll
-      for (int a = 0; a < MTI; ++a) af[a] = (kk < n) ? carry[kk * ldcar + a * 16 + li] : 0.0;
+        for (int a = 0; a < MTI; ++a) af[a] = (kk < n) ? carry[kk * ldcar + a * 16 + li] : 0.0;
 #pragma unroll
-      for (int b = 0; b < NTW; ++b)
-        bf[b] = (kk < n && bcol[b] < n2) ? Qk[(size_t)(n + kk) * n2 + bcol[b]] : 0.0;
+        for (int b = 0; b < NTW; ++b) bf[b] = (kk < n && bcol[b] < n2) ? Qc[(size_t)kk * n2 + bcol[b]] : 0.0;
+      };
+      load(a0, b0, 0);
+      for (int k0 = 0; k0 < n; k0 += 8) {
+        if (k0 + 4 < n) load(a1, b1, k0 + 4);
 #pragma unroll
-      for (int a = 0; a < MTI; ++a)
+        for (int a = 0; a < MTI; ++a)
 #pragma unroll
-        for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(af[a], bf[b], acc[a][b]);
+          for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
+        if (k0 + 8 < n) load(a0, b0, k0 + 8);
+        if (k0 + 4 < n) {
+#pragma unroll
+          for (int a = 0; a < MTI; ++a)
+#pragma unroll
+            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a1[a], b1[b], acc[a][b]);
+        }
+      }
     }
     __syncthreads();
     // epilogue
