@@ -47,148 +47,209 @@ __device__ __forceinline__ d4 d4zero() {
 // ---------------------------------------------------------------------------
 // Sub-panel Householder QR (K1)
 // ---------------------------------------------------------------------------
-// 256 threads.  Lane layout: cg = lane & 3 is the column group (CPG columns
-// each), rg = (lane >> 2) + 16 * wave is the row group; thread (rg, cg)
-// holds rows rg + 64 q (q < RPG) of its CPG columns in registers.  Per
-// column c one fused reduction produces, for every column j of the
-// sub-panel, sum_{r > c} x_r a_rj with x = column c below the pivot:
-//   j == c : ||x||^2          (dlarfg)
-//   j >  c : the dot products of H_c's application
-//   j <  c : v_j^T v_c        (T factor, dlarft)
-// so each column costs one block-wide reduction and one barrier.
-// Reduce v[0..CPG) over the 16 lanes sharing lane & 3 (xor 32, 16, 8, 4).
-// Each stage halves the entries a lane keeps ("transpose reduce"), so CPG
-// values cost CPG - 1 + (stages left) shuffles instead of 4 * CPG.  On
-// return the lane holds the wave sum of entry idx.  Stage sizes are
-// template arguments so every array index is a compile-time constant.
-template <int CNT, int MASK, int CPG>
-__device__ __forceinline__ void transpose_stage(double (&v)[CPG], int lane, int& idx) {
-  if constexpr (MASK >= 4) {
-    if constexpr (CNT >= 2) {
-      constexpr int half = CNT / 2;
-      const bool bit = (lane & MASK) != 0;
-#pragma unroll
-      for (int t = 0; t < half; ++t) {
-        const double send = bit ? v[t] : v[t + half];
-        const double keep = bit ? v[t + half] : v[t];
-        v[t] = keep + __shfl_xor(send, MASK);
-      }
-      if (bit) idx += half;
-      transpose_stage<half, MASK / 2, CPG>(v, lane, idx);
-    } else {
-      v[0] += __shfl_xor(v[0], MASK);
-      transpose_stage<CNT, MASK / 2, CPG>(v, lane, idx);
-    }
-  }
-}
+// 256 threads = 4 waves; wave w owns columns w CPW .. w CPW + CPW - 1 of
+// the sub-panel (NB = 4 CPW columns), lane l holds rows l + 64 q (q < RPG)
+// of them in registers.  Per column c:
+//   * the wave owning c has already updated it and put it in LDS (xs);
+//     one barrier, then every wave reads it (x) at its rows;
+//   * each wave forms, for its own columns j, sum_{r > c} x_r a_rj and
+//     ||x||^2 and reduces them over its 64 lanes (permlane swaps + DPP):
+//       j == c : ||x||^2 (dlarfg),  j > c : H_c's dot products,
+//       j < c  : v_j^T v_c (the T factor, dlarft);
+//   * every wave computes the same dlarfg from the same values, applies
+//     H_c to its columns, and the owner of c + 1 updates that column first
+//     and publishes it for the next step.
+// So a column costs one barrier and no cross-wave reduction.
 
-template <int CPG>
-__device__ __forceinline__ double wave_transpose_reduce(double (&v)[CPG], int lane, int& idx) {
-  idx = 0;
-  transpose_stage<CPG, 32, CPG>(v, lane, idx);
-  return v[0];
-}
-
-__device__ double block_sum_256(double v, double* scratch) {
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) scratch[wave] = v;
-  __syncthreads();
-  const double r = scratch[0] + scratch[1] + scratch[2] + scratch[3];
-  __syncthreads();
-  return r;
-}
-
-__device__ double block_max_256(double v, double* scratch) {
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) scratch[wave] = v;
-  __syncthreads();
-  const double r = fmax(fmax(scratch[0], scratch[1]), fmax(scratch[2], scratch[3]));
-  __syncthreads();
-  return r;
-}
-
-// T of the compact WY form (dlarft, forward, columnwise) from the Gram
-// entries G[j][c] = v_j^T v_c (j < c) and tau; lane i owns row i of T:
-//   T[c][c] = tau_c,  T[i][c] = -tau_c * sum_{j=i}^{c-1} T[i][j] G[j][c].
-// Row i is kept in registers (the loops are unrolled), so the only memory
-// traffic is the broadcast read of G.
-template <int NB>
-__device__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj, double* __restrict__ Tout, int ldt,
-                        int lane) {
-  if (lane >= NB) return;
-  const int i = lane;
-  double Trow[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) Trow[j] = 0.0;
-  for (int c = 0; c < nbj; ++c) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < c) s = fma(Trow[j], G[j][c], s);  // Trow[j] = 0 for j < i
-    const double v = (i < c) ? -taus[c] * s : (i == c ? taus[c] : 0.0);
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j == c) Trow[j] = v;
-  }
-#pragma unroll
-  for (int c = 0; c < NB; ++c)
-    if (c < nbj) Tout[(size_t)c * ldt + i] = (i < nbj) ? Trow[c] : 0.0;
-}
-
-// Broadcast lane (lane & ~3) | cc of each quad to the whole quad (DPP).
-template <int CC>
-__device__ __forceinline__ double quad_bcast_c(double v) {
-  constexpr int ctrl = CC | (CC << 2) | (CC << 4) | (CC << 6);
+// 64-bit DPP move (two 32-bit halves); CTRL is a GFX9 DPP control code.
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v) {
   const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ double quad_bcast(double v, int cc) {
-  switch (cc) {
-    case 0: return quad_bcast_c<0>(v);
-    case 1: return quad_bcast_c<1>(v);
-    case 2: return quad_bcast_c<2>(v);
-    default: return quad_bcast_c<3>(v);
+// Cross-half exchanges of gfx950: with (x', y') = swap(x, y),
+//   permlane32: x'[l] = y[l - 32] for l >= 32, y'[l] = x[l + 32] for l < 32;
+//   permlane16: the same between odd rows of x and even rows of y (rows of
+//   16 lanes).  So x' + y' is, for lanes whose stage bit is clear,
+//   x[l] + x[partner] and, for the others, y[partner] + y[l].
+template <int S>
+__device__ __forceinline__ double swap_add(double x, double y) {
+  const unsigned long long bx = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long by = __builtin_bit_cast(unsigned long long, y);
+  unsigned lo0, lo1, hi0, hi1;
+  if constexpr (S == 0) {
+    const auto l = __builtin_amdgcn_permlane32_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    lo0 = l[0]; lo1 = l[1]; hi0 = h[0]; hi1 = h[1];
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    lo0 = l[0]; lo1 = l[1]; hi0 = h[0]; hi1 = h[1];
+  }
+  const double x2 = __builtin_bit_cast(double, ((unsigned long long)hi0 << 32) | lo0);
+  const double y2 = __builtin_bit_cast(double, ((unsigned long long)hi1 << 32) | lo1);
+  return x2 + y2;
+}
+
+// Value of the partner lane at stages 2..5 (lane ^ 8 by row_ror:8, the
+// mirror inside each 8-lane half which flips bit 2, lane ^ 2, lane ^ 1).
+template <int S>
+__device__ __forceinline__ double dpp_partner(double v) {
+  if constexpr (S == 2) return dpp64<0x128>(v);
+  else if constexpr (S == 3) return dpp64<0x141>(v);
+  else if constexpr (S == 4) return dpp64<0x4E>(v);
+  else return dpp64<0xB1>(v);
+}
+
+// "Transpose reduce" of v[0..CNT) over the 64 lanes of a wave.  Stage S
+// pairs lanes that differ in bit 32 >> S (and hold the same entries); while
+// more than one entry is left a stage halves the entries a lane keeps (the
+// lane with the bit set keeps the upper half), afterwards stages just add.
+// Every lane then holds the full sum of one entry; which one is fixed by its
+// bits of the halving stages (see wave_reduce_lane).
+template <int CNT, int S, int CPW>
+__device__ __forceinline__ void wave_reduce_stage(double (&v)[CPW], int lane) {
+  if constexpr (S < 6) {
+    if constexpr (CNT >= 2) {
+      constexpr int half = CNT / 2;
+      if constexpr (S < 2) {
+#pragma unroll
+        for (int t = 0; t < half; ++t) v[t] = swap_add<S>(v[t], v[t + half]);
+      } else {
+        const bool bit = (lane & (32 >> S)) != 0;
+#pragma unroll
+        for (int t = 0; t < half; ++t) {
+          const double send = bit ? v[t] : v[t + half];
+          const double keep = bit ? v[t + half] : v[t];
+          v[t] = keep + dpp_partner<S>(send);
+        }
+      }
+      wave_reduce_stage<half, S + 1, CPW>(v, lane);
+    } else {
+      if constexpr (S < 2) v[0] = swap_add<S>(v[0], v[0]);
+      else v[0] += dpp_partner<S>(v[0]);
+      wave_reduce_stage<CNT, S + 1, CPW>(v, lane);
+    }
   }
 }
 
-// ||x|| over the block with scaling by max|x| (two block reductions); the
-// rare path of the panel's dlarfg, kept out of line so the unrolled column
-// loop stays small enough for the instruction cache.
-template <int RPG>
-__device__ __noinline__ double scaled_norm_256(const double* x, double* scratch) {
-  double lm = 0.0;
-  for (int q = 0; q < RPG; ++q) lm = fmax(lm, fabs(x[q]));
-  const double amax = block_max_256(lm, scratch);
-  double ls = 0.0;
-  if (amax > 0.0) {
-    for (int q = 0; q < RPG; ++q) {
-      const double y = x[q] / amax;
-      ls = fma(y, y, ls);
-    }
-  }
-  const double s2 = block_sum_256(ls, scratch);
-  return amax > 0.0 ? amax * sqrt(s2) : 0.0;
+// A lane holding the full sum of entry jj after wave_reduce_stage<CPW, 0>:
+// the bits of jj, most significant first, are its bits 32, 16, 8, ...
+template <int CPW>
+__device__ __forceinline__ constexpr int wave_reduce_lane(int jj) {
+  int lane = 0, bitv = 32;
+  for (int h = CPW / 2; h >= 1; h /= 2, bitv /= 2)
+    if (jj & h) lane |= bitv;
+  return lane;
 }
+
+__device__ __forceinline__ double readlane64(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Wave-wide sum and max, the same bits in every lane (lane 0's tree).
+__device__ __forceinline__ double wave_sum(double v, int lane) {
+  double t[1] = {v};
+  wave_reduce_stage<1, 0, 1>(t, lane);
+  return readlane64(t[0], 0);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  return readlane64(v, 0);
+}
+
+// T of the compact WY form (dlarft, forward, columnwise) from the Gram
+// entries G[j][c] = v_j^T v_c (read for j < c only) and tau (zero for the
+// columns at and beyond nbj), by all 256 threads of the workgroup.
+// Recursive doubling: with T1, T2 the factors of two adjacent column
+// groups, H1 H2 = I - V T V^T with
+//   T = [[T1, -T1 (V1^T V2) T2], [0, T2]],
+// so levels b = 1, 2, 4, ... NB/2 each form the off-diagonal blocks of all
+// merges of width 2b in two barrier-separated products.  This equals the
+// column recurrence T[0:c, c] = -tau_c T[0:c, 0:c] V^T v_c of dlarft up to
+// rounding.  Tl and X are LDS scratch of NB x (NB + 1).  Must be called by
+// every thread of the block.  Writes Tout[c * ldt + i] for c < nbj, i < NB
+// (zero for i >= nbj).
+template <int NB>
+__device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj,
+                                        double* __restrict__ Tout, int ldt, double (*Tl)[NB + 1],
+                                        double (*X)[NB + 1]) {
+  static_assert(NB * NB / 4 <= 256, "one thread per entry of the widest level");
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e / NB, j = e % NB;
+    Tl[i][j] = (i == j) ? taus[i] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 1; b < NB; b *= 2) {
+    const int e = tid;
+    const bool act = e < NB * b / 2;
+    const int mi = e / (b * b), loc = e % (b * b);
+    const int r = loc / b, cq = loc % b;
+    const int s0 = mi * 2 * b;
+    if (act) {  // X = (V1^T V2) T2
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < b; ++j) acc = fma(G[s0 + r][s0 + b + j], Tl[s0 + b + j][s0 + b + cq], acc);
+      X[s0 + r][s0 + b + cq] = acc;
+    }
+    __syncthreads();
+    if (act) {  // T12 = -T1 X
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < b; ++j) acc = fma(Tl[s0 + r][s0 + j], X[s0 + j][s0 + b + cq], acc);
+      Tl[s0 + r][s0 + b + cq] = -acc;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < NB * nbj; e += 256) {
+    const int c = e / NB, i = e % NB;
+    Tout[(size_t)c * ldt + i] = (i < nbj) ? Tl[i][c] : 0.0;
+  }
+}
+
+// ||x|| with scaling by max|x|, over one wave (every wave holds all of x):
+// the rare path of the panel's dlarfg, taken when the plain sum of squares
+// leaves the safe range.
+template <int RPG>
+__device__ __forceinline__ double scaled_norm_wave(const double (&x)[RPG], int lane) {
+  double lm = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPG; ++q) lm = fmax(lm, fabs(x[q]));
+  const double amax = wave_max(lm);
+  if (!(amax > 0.0)) return 0.0;
+  double ls = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPG; ++q) {
+    const double y = x[q] / amax;
+    ls = fma(y, y, ls);
+  }
+  return amax * sqrt(wave_sum(ls, lane));
+}
+
+struct Larfg {
+  double tau, beta, scale;
+};
 
 // dlarfg from alpha and ||x||: beta, tau and the scale that turns x into
 // the stored reflector, with LAPACK's safe-minimum rescaling loop.
-__device__ __noinline__ void dlarfg_scalar(double alpha, double xnorm, double* tau_o, double* beta_o,
-                                           double* scale_o) {
+__device__ __forceinline__ Larfg dlarfg_scalar(double alpha, double xnorm) {
   const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-  double tau, beta, scale;
+  Larfg r;
   if (xnorm == 0.0) {
-    tau = 0.0;
-    beta = alpha;
-    scale = 1.0;
+    r.tau = 0.0;
+    r.beta = alpha;
+    r.scale = 1.0;
   } else {
-    beta = -copysign(hypot(alpha, xnorm), alpha);
+    double beta = -copysign(hypot(alpha, xnorm), alpha);
     double al = alpha, sc_acc = 1.0;
     int knt = 0;
     if (fabs(beta) < safmin) {
@@ -201,124 +262,149 @@ __device__ __noinline__ void dlarfg_scalar(double alpha, double xnorm, double* t
       } while (fabs(beta) < safmin && knt < 20);
       beta = -copysign(hypot(al, xnorm * sc_acc), al);
     }
-    tau = (beta - al) / beta;
-    scale = sc_acc / (al - beta);
+    r.tau = (beta - al) / beta;
+    r.scale = sc_acc / (al - beta);
     for (int q2 = 0; q2 < knt; ++q2) beta *= safmin;
+    r.beta = beta;
   }
-  *tau_o = tau;
-  *beta_o = beta;
-  *scale_o = scale;
+  return r;
 }
 
-template <int RPG, int CPG>
+template <int RPG, int CPW>
 __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
                                                            const Item* __restrict__ items) {
-  constexpr int NB = 4 * CPG;
+  constexpr int NB = 4 * CPW;
   const Item it = items[blockIdx.x];
   const PanelTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int cg = lane & 3;
-  const int rg = (lane >> 2) + 16 * wave;  // rows rg + 64 q; only q = 0 rows can lie at or above a pivot (c < 32)
   const int m = t.m, nbj = t.nbj, lda = t.lda;
+  const int col0 = wave * CPW;
 
-  __shared__ double red[2][NB][4];  // [buffer][column][wave] partial sums
-  __shared__ double prow[2][NB];
+  __shared__ double xs[2][RPG][64];  // the next column, double-buffered
   __shared__ double G[NB][NB + 1];
   __shared__ double taus[NB];
-  __shared__ double scratch[4];
+  __shared__ double Tl[NB][NB + 1];
+  __shared__ double Xl[NB][NB + 1];
 
-  double a[RPG][CPG];
+  for (int e = tid; e < NB * (NB + 1); e += 256) (&G[0][0])[e] = 0.0;
+  if (tid < NB) taus[tid] = 0.0;
+  double a[RPG][CPW];
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
-    const int row = rg + 64 * q;
+    const int row = lane + 64 * q;
 #pragma unroll
-    for (int jj = 0; jj < CPG; ++jj) {
-      const int col = cg * CPG + jj;
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int col = col0 + jj;
       a[q][jj] = (row < m && col < nbj) ? t.P[(size_t)col * lda + row] : 0.0;
     }
   }
-
-  for (int cc = 0; cc < 4; ++cc) {
+  if (wave == 0) {
 #pragma unroll
-    for (int jc = 0; jc < CPG; ++jc) {
-      const int c = cc * CPG + jc;
+    for (int q = 0; q < RPG; ++q) xs[0][q][lane] = a[q][0];
+  }
+
+  for (int cw = 0; cw < 4; ++cw) {
+#pragma unroll
+    for (int jc = 0; jc < CPW; ++jc) {
+      const int c = cw * CPW + jc;
       if (c < nbj) {
         const int buf = c & 1;
-        // x = column c below the pivot at my rows (zero at and above it)
+        __syncthreads();
         double x[RPG];
 #pragma unroll
-        for (int q = 0; q < RPG; ++q) x[q] = __shfl(a[q][jc], (lane & ~3) | cc);
-        x[0] = (rg > c) ? x[0] : 0.0;
-        double P[CPG];
+        for (int q = 0; q < RPG; ++q) x[q] = xs[buf][q][lane];
+        const double alpha = readlane64(x[0], c);  // row c < 64 sits in lane c, q = 0
+        x[0] = (lane > c) ? x[0] : 0.0;            // zero at and above the pivot
+        double pr[CPW];                             // pivot row of my columns
 #pragma unroll
-        for (int jj = 0; jj < CPG; ++jj) {
+        for (int jj = 0; jj < CPW; ++jj) pr[jj] = readlane64(a[0][jj], c);
+
+        double P[CPW];
+        double nrm = x[0] * x[0];
+#pragma unroll
+        for (int q = 1; q < RPG; ++q) nrm = fma(x[q], x[q], nrm);
+#pragma unroll
+        for (int jj = 0; jj < CPW; ++jj) {
           double acc = x[0] * a[0][jj];
 #pragma unroll
           for (int q = 1; q < RPG; ++q) acc = fma(x[q], a[q][jj], acc);
           P[jj] = acc;
         }
-        int idx;
-        const double ws = wave_transpose_reduce<CPG>(P, lane, idx);
-        if ((lane & 4) == 0) red[buf][cg * CPG + idx][wave] = ws;
-        if (rg == c) {
+        const double xn2 = wave_sum(nrm, lane);
+        wave_reduce_stage<CPW, 0, CPW>(P, lane);
+        double tot[CPW];
 #pragma unroll
-          for (int jj = 0; jj < CPG; ++jj) prow[buf][cg * CPG + jj] = a[0][jj];
-        }
-        __syncthreads();
+        for (int jj = 0; jj < CPW; ++jj) tot[jj] = readlane64(P[0], wave_reduce_lane<CPW>(jj));
 
-        const double xn2 = (red[buf][c][0] + red[buf][c][1]) + (red[buf][c][2] + red[buf][c][3]);
-        const double alpha = prow[buf][c];
-
-        // dlarfg.  The fast path takes ||x|| = sqrt(sum x^2); a sum of
-        // squares outside [2^-960, 2^960] is recomputed with scaling.
-        const double xnorm = (!(xn2 > 0x1p-960) || !(xn2 < 0x1p960)) ? scaled_norm_256<RPG>(x, scratch) : sqrt(xn2);
+        // dlarfg.  In range, beta = -sign(alpha) sqrt(alpha^2 + ||x||^2)
+        // with one division; a sum of squares outside [2^-960, 2^960]
+        // (including x = 0) or |alpha| >= 2^480 takes LAPACK's scaled path.
         double tau, beta, scale;
-        dlarfg_scalar(alpha, xnorm, &tau, &beta, &scale);
+        if (xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480) {
+          // r = 1 / (beta (alpha - beta)):
+          // tau = (beta - alpha) / beta = -(alpha - beta)^2 r, scale = beta r
+          beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+          const double amb = alpha - beta;
+          const double r = 1.0 / (beta * amb);
+          tau = -(amb * amb) * r;
+          scale = beta * r;
+        } else {
+          const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
+          tau = lf.tau;
+          beta = lf.beta;
+          scale = lf.scale;
+        }
 
         // w_j = tau v_c^T a_j (j > c); G[j][c] = v_j^T v_c (j < c)
-        double w[CPG];
+        double w[CPW];
 #pragma unroll
-        for (int jj = 0; jj < CPG; ++jj) {
-          const int j = cg * CPG + jj;
-          const double tot = (red[buf][j][0] + red[buf][j][1]) + (red[buf][j][2] + red[buf][j][3]);
-          const double d = fma(scale, tot, prow[buf][j]);
+        for (int jj = 0; jj < CPW; ++jj) {
+          const int j = col0 + jj;
+          const double d = fma(scale, tot[jj], pr[jj]);
           w[jj] = (j > c) ? tau * d : 0.0;
-          if (j < c && rg == 0) G[j][c] = d;
+          if (lane == jj) G[j][c] = (j < c) ? d : 0.0;
         }
-        // a <- (I - tau v v^T) a with v = 0 above the pivot, 1 at it, x*scale below
-        const double v0 = (rg > c) ? x[0] * scale : (rg == c ? 1.0 : 0.0);
+        double v[RPG];
+        v[0] = (lane > c) ? x[0] * scale : (lane == c ? 1.0 : 0.0);
 #pragma unroll
-        for (int jj = 0; jj < CPG; ++jj) a[0][jj] = fma(-v0, w[jj], a[0][jj]);
+        for (int q = 1; q < RPG; ++q) v[q] = x[q] * scale;
+        // slot of column c + 1 first; its owner publishes it
+        const int jn = (jc + 1) % CPW;
+        const int wn = cw + (jc + 1) / CPW;
 #pragma unroll
-        for (int q = 1; q < RPG; ++q) {
-          const double vq = x[q] * scale;
+        for (int q = 0; q < RPG; ++q) a[q][jn] = fma(-v[q], w[jn], a[q][jn]);
+        if (wave == wn) {
 #pragma unroll
-          for (int jj = 0; jj < CPG; ++jj) a[q][jj] = fma(-vq, w[jj], a[q][jj]);
+          for (int q = 0; q < RPG; ++q) xs[buf ^ 1][q][lane] = a[q][jn];
         }
-        if (cg == cc) {
-          a[0][jc] = (rg > c) ? v0 : (rg == c ? beta : a[0][jc]);
 #pragma unroll
-          for (int q = 1; q < RPG; ++q) a[q][jc] = x[q] * scale;
+        for (int jj = 0; jj < CPW; ++jj) {
+          if (jj == jn) continue;
+#pragma unroll
+          for (int q = 0; q < RPG; ++q) a[q][jj] = fma(-v[q], w[jj], a[q][jj]);
         }
-        if (tid == 0) {
-          t.tau[c] = tau;
-          taus[c] = tau;
+        if (wave == cw) {  // column c becomes the reflector (beta on the diagonal)
+          a[0][jc] = (lane > c) ? v[0] : (lane == c ? beta : a[0][jc]);
+#pragma unroll
+          for (int q = 1; q < RPG; ++q) a[q][jc] = v[q];
         }
+        if (tid == 0) taus[c] = tau;
       }
     }
   }
 
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
-    const int row = rg + 64 * q;
+    const int row = lane + 64 * q;
 #pragma unroll
-    for (int jj = 0; jj < CPG; ++jj) {
-      const int col = cg * CPG + jj;
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int col = col0 + jj;
       if (row < m && col < nbj) t.P[(size_t)col * lda + row] = a[q][jj];
     }
   }
   __syncthreads();
-  if (wave == 0) build_T<NB>(G, taus, nbj, t.T, t.ldt, lane);
+  if (tid < nbj) t.tau[tid] = taus[tid];
+  build_T<NB>(G, taus, nbj, t.T, t.ldt, Tl, Xl);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,7 +578,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
     G[i][j] = (Gp[0][i][j] + Gp[1][i][j]) + (Gp[2][i][j] + Gp[3][i][j]);
   }
   __syncthreads();
-  if (wave == 0) build_T<32>(G, taus, nbj, t.T, t.ldt, lane);
+  build_T<32>(G, taus, nbj, t.T, t.ldt, &Gp[0][0], &Gp[1][0]);
 }
 
 __global__ void fill_int_kernel(int* __restrict__ p, int count, int value) {
