@@ -175,8 +175,8 @@ __device__ __forceinline__ double wave_max(double v) {
 // merges of width 2b in two barrier-separated products.  This equals the
 // column recurrence T[0:c, c] = -tau_c T[0:c, 0:c] V^T v_c of dlarft up to
 // rounding.  Tl and X are LDS scratch of NB x (NB + 1).  Must be called by
-// every thread of the block.  Writes Tout[c * ldt + i] for c < nbj, i < NB
-// (zero for i >= nbj).
+// every thread of the block.  Writes the nbj x nbj factor, Tout[c * ldt + i]
+// for i, c < nbj (ldt >= nbj may be below NB).
 template <int NB>
 __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj,
                                         double* __restrict__ Tout, int ldt, double (*Tl)[NB + 1],
@@ -210,9 +210,9 @@ __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double*
     }
     __syncthreads();
   }
-  for (int e = tid; e < NB * nbj; e += 256) {
-    const int c = e / NB, i = e % NB;
-    Tout[(size_t)c * ldt + i] = (i < nbj) ? Tl[i][c] : 0.0;
+  for (int e = tid; e < nbj * nbj; e += 256) {
+    const int c = e / nbj, i = e % nbj;
+    Tout[(size_t)c * ldt + i] = Tl[i][c];
   }
 }
 
@@ -424,7 +424,13 @@ __device__ __forceinline__ double vload(const double* __restrict__ V, int ldv, i
 // wave w owns row tiles w, w+4, ... (RT of them, m <= 64 RT).  The C layout
 // of a 16 x 16 accumulator tile is exactly four consecutive B fragments of
 // a K = 16 block (register r = rows 4r..4r+3), so phase 1 multiplies
-// straight out of the accumulators and only V is read from memory.
+// straight out of the accumulators.
+// Global memory is only touched in "row-major lanes" layouts where the 16
+// lanes of a quarter-wave read 16 consecutive rows of one column (128-byte
+// segments): the slab as C[rt*16 + li][4r + lk] and V as
+// V[rt*16 + li][4 kq + lk] (phase 3's A fragments).  A per-wave LDS tile
+// turns them into the accumulator layout and phase 1's V^T fragments, and
+// turns the result back for the store.  All loads are issued up front.
 template <int RT>
 __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
                                                          const Item* __restrict__ items) {
@@ -437,46 +443,78 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   __shared__ double Wp[4][32][17];
   __shared__ double Wf[32][17];
   __shared__ double Tl[32][33];
+  __shared__ double Xt[4][16][33];  // per-wave transpose tile
 
-  // the column this lane serves in B / C fragments
-  const int vcol = it.sub * 16 + li;
-  double* cptr = nullptr;
-  if (vcol < t.w1)
-    cptr = t.C1 + (size_t)vcol * t.ldc;
-  else if (vcol - t.w1 < t.w2)
-    cptr = t.C2 + (size_t)(vcol - t.w1) * t.ldc;
-
+  // the slab column of register r in the row-major-lanes layout
+  double* cp[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int vcol = it.sub * 16 + 4 * r + lk;
+    cp[r] = nullptr;
+    if (vcol < t.w1)
+      cp[r] = t.C1 + (size_t)vcol * t.ldc;
+    else if (vcol - t.w1 < t.w2)
+      cp[r] = t.C2 + (size_t)(vcol - t.w1) * t.ldc;
+  }
   for (int e = tid; e < 32 * 32; e += 256) {
     const int i = e & 31, j = e >> 5;
     Tl[i][j] = (i < nbj && j < nbj) ? t.T[(size_t)j * t.ldt + i] : 0.0;
   }
-  d4 acc[RT];
+  // Registers hold V up front for m <= 512; beyond that V tiles are
+  // loaded when used (same layout).
+  constexpr bool PREV = RT <= 8;
+  double cl[RT][4];              // C[rt*16 + li][4r + lk]
+  double vb[PREV ? RT : 1][8];   // V[rt*16 + li][4kq + lk]
 #pragma unroll
   for (int s = 0; s < RT; ++s) {
-    const int rt = wave + 4 * s;
+    const int row = (wave + 4 * s) * 16 + li;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rt * 16 + crow(lk, r);
-      acc[s][r] = (cptr && row < m) ? cptr[row] : 0.0;
+    for (int r = 0; r < 4; ++r) cl[s][r] = (cp[r] && row < m) ? cp[r][row] : 0.0;
+    if constexpr (PREV) {
+#pragma unroll
+      for (int kq = 0; kq < 8; ++kq) vb[s][kq] = vload(t.V, t.ldv, m, nbj, row, 4 * kq + lk);
     }
   }
-  // phase 1: W = V^T C
+  auto vtile = [&](int s, int kq) -> double {
+    if constexpr (PREV) return vb[s][kq];
+    else return vload(t.V, t.ldv, m, nbj, (wave + 4 * s) * 16 + li, 4 * kq + lk);
+  };
+  double (*X)[33] = Xt[wave];
+
+  // phase 1: W = V^T C, two independent accumulator pairs
   const bool two = nbj > 16;
-  d4 w0 = d4zero(), w1 = d4zero();
+  d4 acc[RT];
+  d4 w0a = d4zero(), w0b = d4zero(), w1a = d4zero(), w1b = d4zero();
 #pragma unroll
   for (int s = 0; s < RT; ++s) {
-    const int rt = wave + 4 * s;
-    if (rt * 16 < m) {
+    // slab tile -> accumulator layout C[4r + lk][li]
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[li][4 * r + lk] = cl[s][r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[s][r] = X[4 * r + lk][li];
+    // V tile -> V^T fragments V[4r + lk][li], V[4r + lk][16 + li]
+#pragma unroll
+    for (int kq = 0; kq < 8; ++kq) X[li][4 * kq + lk] = vtile(s, kq);
+    double va[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      va[r][0] = X[4 * r + lk][li];
+      va[r][1] = X[4 * r + lk][16 + li];
+    }
+    if ((wave + 4 * s) * 16 < m) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = rt * 16 + 4 * r + lk;
-        const double a0 = vload(t.V, t.ldv, m, nbj, row, li);
-        const double a1 = two ? vload(t.V, t.ldv, m, nbj, row, 16 + li) : 0.0;
-        w0 = mfma(a0, acc[s][r], w0);
-        if (two) w1 = mfma(a1, acc[s][r], w1);
+        if (s & 1) {
+          w0b = mfma(va[r][0], acc[s][r], w0b);
+          if (two) w1b = mfma(va[r][1], acc[s][r], w1b);
+        } else {
+          w0a = mfma(va[r][0], acc[s][r], w0a);
+          if (two) w1a = mfma(va[r][1], acc[s][r], w1a);
+        }
       }
     }
   }
+  const d4 w0 = w0a + w0b, w1 = w1a + w1b;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     Wp[wave][crow(lk, r)][li] = w0[r];
@@ -505,29 +543,22 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
     Wf[i][j] = -s2;
   }
   __syncthreads();
-  // phase 3: C += V Wf
+  // phase 3: C += V Wf, then back to the row-major-lanes layout and out
 #pragma unroll
   for (int s = 0; s < RT; ++s) {
     const int rt = wave + 4 * s;
     if (rt * 16 < m) {
 #pragma unroll
-      for (int k0 = 0; k0 < 32; k0 += 4) {
-        if (k0 < nbj) {
-          const double av = vload(t.V, t.ldv, m, nbj, rt * 16 + li, k0 + lk);
-          acc[s] = mfma(av, Wf[k0 + lk][li], acc[s]);
-        }
-      }
+      for (int kq = 0; kq < 8; ++kq)
+        if (4 * kq < nbj) acc[s] = mfma(vtile(s, kq), Wf[4 * kq + lk][li], acc[s]);
     }
-  }
-  if (cptr) {
 #pragma unroll
-    for (int s = 0; s < RT; ++s) {
-      const int rt = wave + 4 * s;
+    for (int r = 0; r < 4; ++r) X[crow(lk, r)][li] = acc[s][r];
+    const int row = rt * 16 + li;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rt * 16 + crow(lk, r);
-        if (row < m) cptr[row] = acc[s][r];
-      }
+    for (int r = 0; r < 4; ++r) {
+      const double v = X[li][4 * r + lk];
+      if (cp[r] && row < m) cp[r][row] = v;
     }
   }
 }

@@ -50,7 +50,9 @@ def make(kind, a, p, seed=7):
 
 PHASE_CASES = [("dqmc", 2, 2), ("dqmc", 2, 3), ("dqmc", 2, 4), ("dqmc", 3, 5), ("dqmc", 4, 8), ("dqmc", 6, 4),
                ("dqmc", 8, 6), ("dqmc", 10, 3), ("rand_eye", 1, 5), ("rand_eye", 17, 4), ("rand_full", 33, 3),
-               ("dqmc", 12, 3), ("rand_eye", 130, 2)]
+               ("dqmc", 12, 3), ("rand_eye", 130, 2),
+               # 2n above 512: the m <= 1024 variants of the panel and reflector kernels
+               ("rand_eye", 300, 3), ("dqmc", 20, 2)]
 
 
 @pytest.mark.parametrize("kind,a,p", PHASE_CASES)
