@@ -175,6 +175,130 @@ def bench_batched(B, lx, p, count, reps, seed):
     return dict(ms=float(np.median(times)), resid=res, info_bad=sum(1 for i in infos if i != 0))
 
 
+def bench_uneven(B, count, seed, reps, tune_p, log_fn):
+    """The uneven mixed workload: `count` matrices with lattice side from
+    {6, 8, 10, 12, 16}, p from U{10..80}, U from U[2, 6] (seeded draw).
+    (1) the autotuner picks nb / scan tile height per n on small batches;
+    (2) all matrices in ONE plan on the tuned context, captured in a HIP
+        graph, `reps` executions on rebuilt inputs;
+    (3) the plain path on the same matrices: one single-matrix plan each on
+        an untuned context, executed back to back.
+    Every result of (2) is residual-checked on the device, and a sample of
+    (3)."""
+    import torch
+
+    from bsofi_amd import autotune, flops, workload
+
+    specs = workload.uneven_specs(count, seed)
+    ctx = B.Context(0)
+    t0 = time.perf_counter()
+    table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=8, reps=2, log=log_fn)
+    tune_s = time.perf_counter() - t0
+    log_fn(f"[bench] uneven: building {count} matrices")
+    mats = workload.DeviceMatrices(B, ctx, specs, seed=seed)
+    tot_flops = sum(flops.bsofi(n, p) for (_, n, p, _, _) in mats.specs)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    plan = B.Plan(ctx, mats.descs())
+    plan.use_graph(True)
+    plan.execute()  # warm-up + graph instantiation
+    ctx.synchronize()
+    times = []
+    for r in range(reps):
+        mats.rebuild()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            plan.execute()
+            e1.record(stream)
+        ctx.synchronize()
+        times.append(e0.elapsed_time(e1))
+    infos = plan.info()
+    res = [mats.residual(k) for k in range(len(mats))]
+    bounds = [100 * n * p * EPS for (_, n, p, _, _) in mats.specs]
+    plan.close()
+    batched_ms = float(np.median(times))
+    # plain path: one plan per matrix on an untuned context
+    mats.rebuild()
+    pctx = B.Context(0)
+    pstream = torch.cuda.ExternalStream(pctx.stream)
+    plans = [B.Plan(pctx, [mats.desc(k)]) for k in range(len(mats))]
+    plans[0].execute()  # warm-up of the kernels, then fresh input
+    pctx.synchronize()
+    mats.rebuild([0])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(pstream):
+        e0.record(pstream)
+        for pl in plans:
+            pl.execute()
+        e1.record(pstream)
+    pctx.synchronize()
+    plain_ms = e0.elapsed_time(e1)
+    pinfo = [pl.info()[0] for pl in plans]
+    pres = [mats.residual(k) for k in range(0, len(mats), 16)]
+    pbounds = [bounds[k] for k in range(0, len(mats), 16)]
+    for pl in plans:
+        pl.close()
+    pctx.close()
+    mats.release()
+    ctx.close()
+    torch.cuda.empty_cache()
+    ns = [n for (_, n, _, _, _) in mats.specs]
+    ps = [p for (_, _, p, _, _) in mats.specs]
+    return dict(batched_ms=batched_ms, plain_ms=plain_ms, tot_flops=tot_flops, tune=table, tune_s=tune_s,
+                resid_ok=all(r <= b for r, b in zip(res, bounds)) and all(i == 0 for i in infos),
+                plain_resid_ok=all(r <= b for r, b in zip(pres, pbounds)) and all(i == 0 for i in pinfo),
+                resid_max_rel=max(r / b for r, b in zip(res, bounds)), n_range=(min(ns), max(ns)),
+                p_range=(min(ps), max(ps)))
+
+
+def bench_serving(B, seed, nchains, budget_gb, slots, max_batch, cap_requests, requests, log_fn):
+    """Serving: `nchains` DQMC chains (n in {64, 144, 256}, p in {20, 40, 80}),
+    each request flips 1-5 % of its chain's HS fields.  (1) capacity: a
+    closed burst of cap_requests requests, completions / wall; (2) Poisson
+    arrivals at 80 % of that capacity, p50 / p99 latency (arrival to
+    completion) and throughput.  Admission by the HBM budget."""
+    from bsofi_amd import serve
+
+    srv = serve.Server(budget_bytes=int(budget_gb * (1 << 30)), slots=slots, max_batch=max_batch)
+    dims = serve.c5_chains(nchains, seed)
+    chains = [srv.add_chain(lx, p, instance=7000 + k) for k, (lx, p) in enumerate(dims)]
+    # warm-up: one request per distinct shape
+    seen = {}
+    for cid, (lx, p) in zip(chains, dims):
+        seen.setdefault((lx, p), cid)
+    serve.run(srv, list(seen.values()), len(seen), 1e9, seed=seed + 1, max_seconds=120)
+    lat_c, _, info_c, wall_c = serve.run(srv, chains, cap_requests, 1e9, seed=seed + 2, max_seconds=120)
+    capacity = len(lat_c) / wall_c
+    rate = 0.8 * capacity
+    log_fn(f"[bench] serving: capacity {capacity:.1f} req/s, Poisson at {rate:.1f} req/s")
+    lat, waits, infos, wall = serve.run(srv, chains, requests, rate, seed=seed + 3, max_seconds=120)
+    srv.close()
+    return dict(capacity=capacity, rate=rate, done=len(lat), wall=wall, p50=float(np.percentile(lat, 50)) * 1e3,
+                p99=float(np.percentile(lat, 99)) * 1e3, qwait_p50=float(np.percentile(waits, 50)) * 1e3,
+                info_bad=sum(1 for v in list(infos.values()) + list(info_c.values()) if v != 0),
+                shapes=sorted(set(dims)))
+
+
+def bench_e2e(B, lx, p, seed):
+    """C2 through the host-array handle API (hybridxbsoftrih_ + hybridxbsoih_):
+    the PCIe-inclusive time of one inversion (A copied in, G copied out)."""
+    from bsofi_amd import dqmc
+
+    A0, meta = dqmc.dqmc_matrix(lx, p, seed=seed, instance=0)
+    n, N = meta["n"], meta["n"] * p
+    A = np.asfortranarray(A0)
+    tau = np.zeros(N)
+    H = B.HandleAPI(n, p, N)
+    work = np.zeros(max(1, H.lwork()))
+    t0 = time.perf_counter()
+    i1 = H.bsoftri(A, tau, work)
+    i2 = H.bsoi(A, tau, work)
+    dt = time.perf_counter() - t0
+    H.finalize()
+    return dict(seconds=dt, info=(i1, i2))
+
+
 def cpu_baseline(lx, p, seed, threads):
     """The reference's CPU path (oracle/_ref) on the same matrix (instance 0)."""
     import oracle
@@ -207,6 +331,10 @@ def main():
     ap.add_argument("--p", type=int, default=64)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--batched", type=int, default=1, help="also run the 512 x (n=64,p=32) batch")
+    ap.add_argument("--uneven", type=int, default=256, help="matrices in the uneven batch (0 = skip)")
+    ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--e2e", type=int, default=1, help="also time one inversion through the host-array H-API")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -282,6 +410,49 @@ def main():
             "speedup_vs_plain": round(b_ips / plain_ips, 2),
             "resid_max_sampled": b["resid"],
             "info_nonzero": b["info_bad"],
+        }
+    if args.uneven:
+        u = bench_uneven(B, args.uneven, args.seed, 2, 24, log)
+        b_ips = args.uneven / (u["batched_ms"] * 1e-3)
+        p_ips = args.uneven / (u["plain_ms"] * 1e-3)
+        line["uneven"] = {
+            "workload": f"{args.uneven} matrices, lattice side in {{6,8,10,12,16}} (n={u['n_range'][0]}.."
+                        f"{u['n_range'][1]}), p in U{{10..80}} (drawn {u['p_range'][0]}..{u['p_range'][1]}), "
+                        "U in U[2,6]; one plan, HIP graph, autotuned nb / scan tiles",
+            "inversions_per_s": round(b_ips, 2),
+            "ms_per_batch": round(u["batched_ms"], 2),
+            "gflops": round(u["tot_flops"] / (u["batched_ms"] * 1e-3) / 1e9, 1),
+            "frac_fp64_peak": round(u["tot_flops"] / (u["batched_ms"] * 1e-3) / (FP64_PEAK_TFLOPS * 1e12), 4),
+            "plain_inversions_per_s": round(p_ips, 2),
+            "plain_ms_all": round(u["plain_ms"], 1),
+            "speedup_vs_plain": round(b_ips / p_ips, 2),
+            "resid_ok": u["resid_ok"],
+            "plain_resid_ok_sampled": u["plain_resid_ok"],
+            "resid_max_over_bound": u["resid_max_rel"],
+            "autotune": {str(k): f"nb={v['nb']} scan_mti={v['scan_mti']}" for k, v in u["tune"].items()},
+            "autotune_s": round(u["tune_s"], 1),
+        }
+    if args.serve:
+        sv = bench_serving(B, args.seed, 64, 200, 4, 16, max(64, args.serve_requests // 2), args.serve_requests, log)
+        line["serving"] = {
+            "workload": "64 DQMC chains, n in {64,144,256}, p in {20,40,80}; each request flips 1-5% of its "
+                        "chain's HS fields; Poisson arrivals at 80% of the measured capacity; 200 GB HBM budget",
+            "capacity_req_per_s": round(sv["capacity"], 2),
+            "rate_req_per_s": round(sv["rate"], 2),
+            "throughput_req_per_s": round(sv["done"] / sv["wall"], 2),
+            "p50_ms": round(sv["p50"], 2),
+            "p99_ms": round(sv["p99"], 2),
+            "queue_wait_p50_ms": round(sv["qwait_p50"], 2),
+            "requests": sv["done"],
+            "info_nonzero": sv["info_bad"],
+        }
+    if args.e2e:
+        e = bench_e2e(B, args.lx, args.p, args.seed)
+        line["e2e_host_api"] = {
+            "path": "hybridxbsoftrih_ + hybridxbsoih_ on host arrays (A in, X out, G out over PCIe)",
+            "seconds": round(e["seconds"], 3),
+            "inversions_per_s": round(1.0 / e["seconds"], 3),
+            "info": list(e["info"]),
         }
     if args.cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -86,6 +86,9 @@ def _load():
         "bsofi_ctx_set_stream": (i, [_vp, _vp]),
         "bsofi_ctx_stream": (_vp, [_vp]),
         "bsofi_ctx_synchronize": (i, [_vp]),
+        "bsofi_ctx_set_tuning": (i, [_vp, i, i, i]),
+        "bsofi_ctx_get_tuning": (i, [_vp, i, C.POINTER(i), C.POINTER(i)]),
+        "bsofi_ctx_clear_tuning": (i, [_vp]),
         "bsofi_workspace_bytes": (C.c_size_t, [i, i]),
         "bsofi_plan_workspace_bytes": (C.c_size_t, [i, C.POINTER(bsofi_matrix_t), i]),
         "bsofi_plan_create": (i, [_vp, i, C.POINTER(bsofi_matrix_t), i, _vp, C.POINTER(_vp)]),
@@ -253,6 +256,22 @@ class Context:
         rc = lib.bsofi_ctx_synchronize(self.h)
         if rc:
             raise RuntimeError(f"synchronize failed ({rc})")
+
+    def set_tuning(self, n, nb=0, scan_mti=0):
+        """Sub-panel width and scan rows/16 for block size n (0 = default)."""
+        rc = lib.bsofi_ctx_set_tuning(self.h, n, nb, scan_mti)
+        if rc:
+            raise ValueError(f"unsupported tuning n={n} nb={nb} scan_mti={scan_mti} ({rc})")
+
+    def get_tuning(self, n):
+        nb, mti = C.c_int(0), C.c_int(0)
+        rc = lib.bsofi_ctx_get_tuning(self.h, n, C.byref(nb), C.byref(mti))
+        if rc:
+            raise RuntimeError(f"get_tuning failed ({rc})")
+        return nb.value, mti.value
+
+    def clear_tuning(self):
+        lib.bsofi_ctx_clear_tuning(self.h)
 
     def close(self):
         if self.h:

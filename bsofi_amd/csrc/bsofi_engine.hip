@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -68,19 +69,44 @@ struct Geo {
 
 static int nb_for(int n) { return (2 * n <= 512) ? 32 : (2 * n <= 1024 ? 16 : 8); }
 
-static Geo geometry(int n, int p, int lda) {
+// Choices a context can override per block size n (the autotuner's knobs):
+// the sub-panel width nb of BSOF / BSOI and the rows per workgroup of the
+// BSOI scan (16 * scan_mti).  Zero keeps the default.
+struct Tune {
+  int nb = 0, scan_mti = 0;
+};
+
+// Sub-panel widths with a panel_factor_kernel instance for 2n rows.
+static bool nb_supported(int n, int nb) {
+  const int m = 2 * n;
+  if (nb == 32) return m <= 512;
+  if (nb == 16) return m <= 1024;
+  if (nb == 8) return m <= 2048;
+  return false;
+}
+
+static int scan_ntw_for(int n) { return ((2 * n + 15) / 16 + 3) / 4; }
+
+static bool scan_variant_ok(int mti, int ntw) {
+  if (mti == 1) return ntw >= 1 && ntw <= 16;
+  if (mti == 2) return ntw >= 1 && ntw <= 8;
+  if (mti == 4) return ntw >= 1 && ntw <= 4;
+  return false;
+}
+
+static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   Geo g;
   g.n = n;
   g.p = p;
   g.lda = lda;
   g.N = n * p;
-  g.nb = nb_for(n);
+  g.nb = (tune && tune->nb) ? tune->nb : nb_for(n);
   g.S = (2 * n + g.nb - 1) / g.nb;
   const int m = 2 * n;
-  g.panel_variant = m <= 128 ? 0 : m <= 256 ? 1 : m <= 512 ? 2 : m <= 1024 ? 3 : 4;
-  g.scan_mti = n <= 256 ? 2 : 1;
-  const int tiles = (2 * n + 15) / 16;
-  g.scan_ntw = (tiles + 3) / 4;
+  const int rpg = m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
+  g.panel_variant = rpg * 100 + g.nb / 4;  // panel_factor_kernel<RPG, CPW = nb / 4>
+  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti : (n <= 256 ? 2 : 1);
+  g.scan_ntw = scan_ntw_for(n);
   return g;
 }
 
@@ -101,7 +127,8 @@ static WsLayout ws_layout(const Geo& g, int phases) {
     return o;
   };
   const bool need_T = phases & (BSOFI_PHASE_BSOF | BSOFI_PHASE_BSOI);
-  L.T = need_T ? take((size_t)(g.p - 1) * g.S * g.nb * g.nb) : 0;
+  // sized for any sub-panel width nb <= 32: S nb^2 <= 2n nb + nb^2
+  L.T = need_T ? take((size_t)(g.p - 1) * (64 * (size_t)g.n + 1024)) : 0;
   L.Q = (phases & BSOFI_PHASE_BSOI) ? take((size_t)(g.p - 1) * 4 * n2) : 0;
   if (phases & BSOFI_PHASE_BSTRI) {
     L.W = take((size_t)(g.p - 1) * n2);
@@ -121,6 +148,7 @@ struct bsofi_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  std::map<int, bsofi::Tune> tune;  // per block size n
 };
 
 struct bsofi_plan_s {
@@ -205,8 +233,8 @@ struct Planner {
       rounds = std::max(rounds, seq[i].size());
     }
     for (size_t r = 0; r < rounds; ++r) {
-      std::vector<PanelTask> pt[5];
-      std::vector<Item> pi[5];
+      std::map<int, std::vector<PanelTask>> pt;
+      std::map<int, std::vector<Item>> pi;
       std::vector<ReflTask> rt;
       std::vector<Item> ri;
       for (size_t i = 0; i < mats.size(); ++i) {
@@ -246,7 +274,7 @@ struct Planner {
           rt.push_back(q);
         }
       }
-      for (int v = 0; v < 5; ++v) emit(F_PANEL, v, pt[v], pi[v]);
+      for (auto& kv : pt) emit(F_PANEL, kv.first, kv.second, pi[kv.first]);
       emit(F_REFL, refl_variant(rt), rt, ri);
     }
   }
@@ -490,25 +518,23 @@ static void set_scan_attr(size_t shmem) {
                               (int)shmem);
 }
 
-#define SCAN_SWITCH(FN, ...)                                  \
-  switch (variant) {                                          \
-    case 201: FN<2, 1>(__VA_ARGS__); break;                   \
-    case 202: FN<2, 2>(__VA_ARGS__); break;                   \
-    case 203: FN<2, 3>(__VA_ARGS__); break;                   \
-    case 204: FN<2, 4>(__VA_ARGS__); break;                   \
-    case 205: FN<2, 5>(__VA_ARGS__); break;                   \
-    case 206: FN<2, 6>(__VA_ARGS__); break;                   \
-    case 207: FN<2, 7>(__VA_ARGS__); break;                   \
-    case 208: FN<2, 8>(__VA_ARGS__); break;                   \
-    case 109: FN<1, 9>(__VA_ARGS__); break;                   \
-    case 110: FN<1, 10>(__VA_ARGS__); break;                  \
-    case 111: FN<1, 11>(__VA_ARGS__); break;                  \
-    case 112: FN<1, 12>(__VA_ARGS__); break;                  \
-    case 113: FN<1, 13>(__VA_ARGS__); break;                  \
-    case 114: FN<1, 14>(__VA_ARGS__); break;                  \
-    case 115: FN<1, 15>(__VA_ARGS__); break;                  \
-    case 116: FN<1, 16>(__VA_ARGS__); break;                  \
-    default: break;                                           \
+// scan instances: variant = MTI * 100 + NTW (see scan_variant_ok)
+#define SCAN_C(FN, M, W, ...) \
+  case M * 100 + W: FN<M, W>(__VA_ARGS__); break;
+#define SCAN_SWITCH(FN, ...)                                                                                 \
+  switch (variant) {                                                                                         \
+    SCAN_C(FN, 1, 1, __VA_ARGS__) SCAN_C(FN, 1, 2, __VA_ARGS__) SCAN_C(FN, 1, 3, __VA_ARGS__)                \
+    SCAN_C(FN, 1, 4, __VA_ARGS__) SCAN_C(FN, 1, 5, __VA_ARGS__) SCAN_C(FN, 1, 6, __VA_ARGS__)                \
+    SCAN_C(FN, 1, 7, __VA_ARGS__) SCAN_C(FN, 1, 8, __VA_ARGS__) SCAN_C(FN, 1, 9, __VA_ARGS__)                \
+    SCAN_C(FN, 1, 10, __VA_ARGS__) SCAN_C(FN, 1, 11, __VA_ARGS__) SCAN_C(FN, 1, 12, __VA_ARGS__)             \
+    SCAN_C(FN, 1, 13, __VA_ARGS__) SCAN_C(FN, 1, 14, __VA_ARGS__) SCAN_C(FN, 1, 15, __VA_ARGS__)             \
+    SCAN_C(FN, 1, 16, __VA_ARGS__)                                                                           \
+    SCAN_C(FN, 2, 1, __VA_ARGS__) SCAN_C(FN, 2, 2, __VA_ARGS__) SCAN_C(FN, 2, 3, __VA_ARGS__)                \
+    SCAN_C(FN, 2, 4, __VA_ARGS__) SCAN_C(FN, 2, 5, __VA_ARGS__) SCAN_C(FN, 2, 6, __VA_ARGS__)                \
+    SCAN_C(FN, 2, 7, __VA_ARGS__) SCAN_C(FN, 2, 8, __VA_ARGS__)                                              \
+    SCAN_C(FN, 4, 1, __VA_ARGS__) SCAN_C(FN, 4, 2, __VA_ARGS__) SCAN_C(FN, 4, 3, __VA_ARGS__)                \
+    SCAN_C(FN, 4, 4, __VA_ARGS__)                                                                            \
+    default: break;                                                                                          \
   }
 
 static void run_launch(const Launch& L, const char* base, hipStream_t s) {
@@ -517,13 +543,16 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     case F_PANEL: {
       auto t = (const PanelTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
+      // variant = RPG * 100 + CPW
+#define PANEL_CASE(R, C) \
+  case R * 100 + C: hipLaunchKernelGGL((panel_factor_kernel<R, C>), dim3(L.grid), dim3(256), 0, s, t, it); break;
       switch (L.variant) {
-        case 0: hipLaunchKernelGGL((panel_factor_kernel<2, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 1: hipLaunchKernelGGL((panel_factor_kernel<4, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 2: hipLaunchKernelGGL((panel_factor_kernel<8, 8>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 3: hipLaunchKernelGGL((panel_factor_kernel<16, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 4: hipLaunchKernelGGL((panel_factor_kernel<32, 2>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        PANEL_CASE(2, 8) PANEL_CASE(4, 8) PANEL_CASE(8, 8)
+        PANEL_CASE(2, 4) PANEL_CASE(4, 4) PANEL_CASE(8, 4) PANEL_CASE(16, 4)
+        PANEL_CASE(2, 2) PANEL_CASE(4, 2) PANEL_CASE(8, 2) PANEL_CASE(16, 2) PANEL_CASE(32, 2)
+        default: break;
       }
+#undef PANEL_CASE
       break;
     }
     case F_REFL: {
@@ -562,18 +591,45 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
   }
 }
 
-static int scan_variant_ok(int mti, int ntw) {
-  if (mti == 2) return ntw >= 1 && ntw <= 8;
-  if (mti == 1) return ntw >= 9 && ntw <= 16;
-  return 0;
-}
-
 }  // namespace bsofi
+
+static const bsofi::Tune* ctx_tune(const bsofi_ctx_s* ctx, int n) {
+  auto it = ctx->tune.find(n);
+  return it == ctx->tune.end() ? nullptr : &it->second;
+}
 
 // ===========================================================================
 // C API
 // ===========================================================================
 extern "C" {
+
+int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti) {
+  if (!ctx || n < 1 || n > BSOFI_MAX_N || nb < 0 || scan_mti < 0) return -1;
+  Tune t;
+  t.nb = nb;
+  t.scan_mti = scan_mti;
+  const Geo g = geometry(n, 2, 2 * n, &t);
+  if (!nb_supported(n, g.nb) || !scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+  if (nb == 0 && scan_mti == 0)
+    ctx->tune.erase(n);
+  else
+    ctx->tune[n] = t;
+  return 0;
+}
+
+int bsofi_ctx_get_tuning(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti) {
+  if (!ctx || !nb || !scan_mti || n < 1) return -1;
+  const Geo g = geometry(n, 2, 2 * n, ctx_tune(ctx, n));
+  *nb = g.nb;
+  *scan_mti = g.scan_mti;
+  return 0;
+}
+
+int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx) {
+  if (!ctx) return -1;
+  ctx->tune.clear();
+  return 0;
+}
 
 int bsofi_device_count(void) {
   int c = 0;
@@ -663,8 +719,8 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     int a = bsofi_check_args(mats[i].n, mats[i].p, mats[i].lda);
     if (a) return a;
     if (!mats[i].A || !mats[i].tau) return -3;
-    const Geo g = geometry(mats[i].n, mats[i].p, mats[i].lda);
-    if (!scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+    const Geo g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
+    if (!nb_supported(g.n, g.nb) || !scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
   }
   HIPCHK(hipSetDevice(ctx->device));
   auto* P = new bsofi_plan_s();
@@ -685,7 +741,7 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   size_t off = al256((size_t)count * sizeof(int));
   for (int i = 0; i < count; ++i) {
     MatJob& J = jobs[i];
-    J.g = geometry(mats[i].n, mats[i].p, mats[i].lda);
+    J.g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
     J.A = mats[i].A;
     J.tau = mats[i].tau;
     J.info = P->d_info + i;

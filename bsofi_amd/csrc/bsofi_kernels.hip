@@ -319,10 +319,25 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) pr[jj] = readlane64(a[0][jj], c);
 
-        double P[CPW];
+        // ||x||^2 and dlarfg come first and without a branch, so their
+        // dependent chain can interleave with the dot products below.
         double nrm = x[0] * x[0];
 #pragma unroll
         for (int q = 1; q < RPG; ++q) nrm = fma(x[q], x[q], nrm);
+        const double xn2 = wave_sum(nrm, lane);
+        // dlarfg.  In range, beta = -sign(alpha) sqrt(alpha^2 + ||x||^2)
+        // and, with r = 1 / (beta (alpha - beta)),
+        //   tau = (beta - alpha) / beta = -(alpha - beta)^2 r,  scale = beta r.
+        // A sum of squares outside [2^-960, 2^960] (including x = 0) or
+        // |alpha| >= 2^480 takes LAPACK's scaled path instead.
+        const bool inrange = xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480;
+        double beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+        const double amb = alpha - beta;
+        const double rr = 1.0 / (beta * amb);
+        double tau = -(amb * amb) * rr;
+        double scale = beta * rr;
+
+        double P[CPW];
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) {
           double acc = x[0] * a[0][jj];
@@ -330,25 +345,11 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
           for (int q = 1; q < RPG; ++q) acc = fma(x[q], a[q][jj], acc);
           P[jj] = acc;
         }
-        const double xn2 = wave_sum(nrm, lane);
         wave_reduce_stage<CPW, 0, CPW>(P, lane);
         double tot[CPW];
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) tot[jj] = readlane64(P[0], wave_reduce_lane<CPW>(jj));
-
-        // dlarfg.  In range, beta = -sign(alpha) sqrt(alpha^2 + ||x||^2)
-        // with one division; a sum of squares outside [2^-960, 2^960]
-        // (including x = 0) or |alpha| >= 2^480 takes LAPACK's scaled path.
-        double tau, beta, scale;
-        if (xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480) {
-          // r = 1 / (beta (alpha - beta)):
-          // tau = (beta - alpha) / beta = -(alpha - beta)^2 r, scale = beta r
-          beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
-          const double amb = alpha - beta;
-          const double r = 1.0 / (beta * amb);
-          tau = -(amb * amb) * r;
-          scale = beta * r;
-        } else {
+        if (!inrange) {
           const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
           tau = lf.tau;
           beta = lf.beta;

@@ -213,3 +213,38 @@ def test_serving_alone_equals_mix(B, eps):
     s[flips] *= -1
     A = dqmc.assemble(m["expk"], s.reshape(8, 16), m["lam"], 8)
     assert oracle.residual(A, got) <= 100 * 128 * eps
+
+
+def test_tuned_variants_match_oracle(B, eps):
+    """Every (nb, scan_mti) pair the autotuner may pick gives the oracle's
+    inverse: the nb = 16 panel instances, the 64-row scan tiles, and the
+    16-row scan tiles below n = 256."""
+    import torch
+
+    from bsofi_amd import autotune
+
+    O = oracle.Oracle("auto", threads=1)
+    ctx = B.Context(0)
+    for lx, p in [(4, 5), (8, 4), (10, 3), (12, 3)]:
+        n = lx * lx
+        A0, _ = dqmc.dqmc_matrix(lx, p, instance=70 + lx)
+        G = A0.copy(order="F")
+        O.invert(n, p, G)
+        for nb, mti in autotune.candidates(n):
+            print(f"tuned case n={n} p={p} nb={nb} scan_mti={mti}", flush=True)
+            ctx.set_tuning(n, nb, mti)
+            assert ctx.get_tuning(n) == (nb, mti)
+            d = to_dev(A0)
+            t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+            got = from_dev(d, n * p)
+            assert rel(got, G) <= 1e-10, (n, nb, mti)
+            assert oracle.residual(A0, got) <= 100 * n * p * eps
+    with pytest.raises(ValueError):
+        ctx.set_tuning(300, 32, 0)  # 2n > 512 has no 32-wide panel
+    with pytest.raises(ValueError):
+        ctx.set_tuning(144, 0, 4)  # 64-row scan tiles need n <= 128
+    ctx.clear_tuning()
+    assert ctx.get_tuning(64) == (32, 2)
+    ctx.close()
