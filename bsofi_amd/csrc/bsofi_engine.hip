@@ -149,16 +149,82 @@ struct bsofi_ctx_s {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::map<int, bsofi::Tune> tune;  // per block size n
+  // Device buffers of destroyed plans (task blobs, workspaces), reused by
+  // later plans on this context so that steady-state plan churn calls
+  // neither hipMalloc nor hipFree (which synchronises the whole device).
+  // A buffer is cached only after the context's stream has drained, and
+  // reuse is ordered on that same stream.
+  std::vector<std::pair<size_t, char*>> cache;
+  size_t cache_bytes = 0;
 };
+
+namespace bsofi {
+static constexpr size_t kCacheMaxEntries = 8;
+
+// Sizes on a ladder of 8 steps per octave, so buffers of similar plans
+// share a size and fit each other.
+static size_t ladder(size_t bytes) {
+  if (bytes <= 4096) return 4096;
+  int e = 63 - __builtin_clzll((unsigned long long)bytes);  // 2^e <= bytes
+  const size_t step = (size_t)1 << (e - 3);
+  return (bytes + step - 1) / step * step;
+}
+
+// Smallest cached buffer of at least `bytes` (and at most 2x, so a huge
+// buffer does not serve a tiny request), else a new allocation.  On return
+// `bytes` is the size of the block, which ctx_free takes back.
+static char* ctx_alloc(bsofi_ctx_s* c, size_t& bytes) {
+  size_t best = SIZE_MAX;
+  int bi = -1;
+  for (size_t i = 0; i < c->cache.size(); ++i) {
+    const size_t b = c->cache[i].first;
+    if (b >= bytes && b <= 2 * bytes + 4096 && b < best) {
+      best = b;
+      bi = (int)i;
+    }
+  }
+  if (bi >= 0) {
+    char* ptr = c->cache[bi].second;
+    c->cache_bytes -= best;
+    c->cache.erase(c->cache.begin() + bi);
+    bytes = best;
+    return ptr;
+  }
+  bytes = ladder(bytes);
+  char* ptr = nullptr;
+  if (hipMalloc((void**)&ptr, bytes) != hipSuccess) return nullptr;
+  return ptr;
+}
+
+// `bytes` is the block size ctx_alloc reported.
+static void ctx_free(bsofi_ctx_s* c, char* ptr, size_t bytes) {
+  if (!ptr) return;
+  c->cache.push_back({bytes, ptr});
+  c->cache_bytes += bytes;
+  while (c->cache.size() > kCacheMaxEntries) {  // drop the oldest
+    (void)hipFree(c->cache.front().second);
+    c->cache_bytes -= c->cache.front().first;
+    c->cache.erase(c->cache.begin());
+  }
+}
+
+static void ctx_trim(bsofi_ctx_s* c) {
+  for (auto& e : c->cache) (void)hipFree(e.second);
+  c->cache.clear();
+  c->cache_bytes = 0;
+}
+}  // namespace bsofi
 
 struct bsofi_plan_s {
   bsofi_ctx_s* ctx = nullptr;
   std::vector<Launch> launches;
   Blob blob;
   char* d_blob = nullptr;
+  size_t blob_bytes = 0;
   char* ws = nullptr;
   bool own_ws = false;
   size_t ws_bytes = 0;
+  size_t ws_block = 0;  // size of the owned workspace block
   int* d_info = nullptr;
   int count = 0;
   bool use_graph = false;
@@ -672,22 +738,33 @@ int bsofi_ctx_create(bsofi_ctx_t* ctx, int device, void* hip_stream) {
 int bsofi_ctx_destroy(bsofi_ctx_t c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
-  if (c->own_stream) {
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipStreamDestroy(c->stream);
-  }
+  (void)hipStreamSynchronize(c->stream);
+  ctx_trim(c);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
 int bsofi_ctx_set_stream(bsofi_ctx_t c, void* s) {
   if (!c) return -1;
+  (void)hipSetDevice(c->device);
+  // cached buffers were last used on the old stream: let it drain first
+  (void)hipStreamSynchronize(c->stream);
   if (c->own_stream) {
-    (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
     c->own_stream = false;
   }
   c->stream = (hipStream_t)s;
+  return 0;
+}
+
+size_t bsofi_ctx_cached_bytes(bsofi_ctx_t c) { return c ? c->cache_bytes : 0; }
+
+int bsofi_ctx_release_cache(bsofi_ctx_t c) {
+  if (!c) return -1;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  ctx_trim(c);
   return 0;
 }
 
@@ -730,7 +807,9 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   if (workspace) {
     P->ws = (char*)workspace;
   } else if (P->ws_bytes) {
-    if (hipMalloc((void**)&P->ws, P->ws_bytes) != hipSuccess) {
+    P->ws_block = P->ws_bytes;
+    P->ws = ctx_alloc(ctx, P->ws_block);
+    if (!P->ws) {
       delete P;
       return BSOFI_ERR_ALLOC;
     }
@@ -776,17 +855,18 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
       SCAN_SWITCH(set_scan_attr, L.shmem)
     }
   }
-  const size_t bytes = std::max<size_t>(P->blob.h.size(), 256);
-  if (hipMalloc((void**)&P->d_blob, bytes) != hipSuccess) {
-    if (P->own_ws) (void)hipFree(P->ws);
+  P->blob_bytes = std::max<size_t>(P->blob.h.size(), 256);
+  P->d_blob = ctx_alloc(ctx, P->blob_bytes);
+  if (!P->d_blob) {
+    if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
     return BSOFI_ERR_ALLOC;
   }
   if (!P->blob.h.empty() &&
       !hip_ok(hipMemcpyAsync(P->d_blob, P->blob.h.data(), P->blob.h.size(), hipMemcpyHostToDevice, ctx->stream),
               "hipMemcpyAsync(plan)")) {
-    (void)hipFree(P->d_blob);
-    if (P->own_ws) (void)hipFree(P->ws);
+    ctx_free(ctx, P->d_blob, P->blob_bytes);
+    if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
     return BSOFI_ERR_NO_DEVICE;
   }
@@ -889,8 +969,9 @@ int bsofi_plan_destroy(bsofi_plan_t P) {
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
   for (auto e : P->ev) (void)hipEventDestroy(e);
-  if (P->d_blob) (void)hipFree(P->d_blob);
-  if (P->own_ws && P->ws) (void)hipFree(P->ws);
+  // the stream has drained: the buffers go back to the context for reuse
+  ctx_free(P->ctx, P->d_blob, P->blob_bytes);
+  if (P->own_ws) ctx_free(P->ctx, P->ws, P->ws_block);
   delete P;
   return 0;
 }
@@ -917,8 +998,9 @@ int bsofi_pcyclic_residual(bsofi_ctx_t ctx, const double* dblocks, const double*
                            double* resid) {
   if (!ctx || !resid) return -1;
   HIPCHK(hipSetDevice(ctx->device));
-  double* acc = nullptr;
-  HIPCHK(hipMalloc((void**)&acc, 3 * sizeof(double)));
+  size_t acc_bytes = 256;
+  double* acc = (double*)ctx_alloc(ctx, acc_bytes);
+  if (!acc) return BSOFI_ERR_ALLOC;
   hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, (int*)acc, 6, 0);
   const int N = n * p;
   const int tiles = p * ((n + 63) / 64) * ((N + 63) / 64);
@@ -928,7 +1010,7 @@ int bsofi_pcyclic_residual(bsofi_ctx_t ctx, const double* dblocks, const double*
   double h[3] = {0, 0, 0};
   hipError_t e = hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(acc);
+  ctx_free(ctx, (char*)acc, acc_bytes);
   if (!hip_ok(e, "residual")) return BSOFI_ERR_NO_DEVICE;
   *resid = std::sqrt(h[0]) / (std::sqrt(h[1]) * std::sqrt(h[2]));
   return 0;

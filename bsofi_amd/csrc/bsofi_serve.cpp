@@ -103,6 +103,12 @@ size_t work_bytes(int n, int p) {
 struct bsofi_server_s {
   int device = 0;
   size_t budget = 0, used = 0;
+  // Freed request buffers (G, tau, field snapshots), keyed by exact size:
+  // chains have fixed shapes, so steady-state serving reuses them and never
+  // calls hipMalloc / hipFree (hipFree synchronises the whole device and
+  // would stall every slot in flight).  Counted against the budget.
+  std::map<size_t, std::vector<void*>> pool;
+  size_t pool_bytes = 0;
   int max_batch = 64;
   std::vector<Lattice> lattices;
   std::vector<Chain> chains;
@@ -113,6 +119,37 @@ struct bsofi_server_s {
   long long completed = 0;
   std::mutex mu;
 };
+
+static void* pool_get(bsofi_server_t s, size_t bytes) {
+  auto it = s->pool.find(bytes);
+  if (it != s->pool.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    s->pool_bytes -= bytes;
+    return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+static void pool_put(bsofi_server_t s, void* p, size_t bytes) {
+  if (!p) return;
+  s->pool[bytes].push_back(p);
+  s->pool_bytes += bytes;
+}
+
+// Free pooled blocks (largest first) until `extra` more bytes fit.
+static void pool_trim(bsofi_server_t s, size_t extra) {
+  while (s->pool_bytes > 0 && s->used + s->pool_bytes + extra > s->budget) {
+    auto it = std::prev(s->pool.end());
+    while (it->second.empty() && it != s->pool.begin()) --it;
+    if (it->second.empty()) break;
+    (void)hipFree(it->second.back());
+    it->second.pop_back();
+    s->pool_bytes -= it->first;
+  }
+}
 
 extern "C" {
 
@@ -153,6 +190,8 @@ int bsofi_server_destroy(bsofi_server_t s) {
     if (r.dtau) (void)hipFree(r.dtau);
     if (r.dfields) (void)hipFree(r.dfields);
   }
+  for (auto& kv : s->pool)
+    for (void* p : kv.second) (void)hipFree(p);
   for (auto& L : s->lattices)
     if (L.d_expk) (void)hipFree(L.d_expk);
   delete s;
@@ -224,23 +263,29 @@ static int launch_slot(bsofi_server_t s, Slot& sl, std::vector<Request*>& batch)
   std::vector<bsofi_matrix_t> mats(batch.size());
   for (size_t i = 0; i < batch.size(); ++i) {
     Request& r = *batch[i];
-    const Chain& c = s->chains[r.chain];
     const size_t N = (size_t)r.n * r.p;
-    if (hipMalloc((void**)&r.dA, N * N * 8) != hipSuccess || hipMalloc((void**)&r.dtau, N * 8) != hipSuccess ||
-        hipMalloc((void**)&r.dfields, r.fields.size()) != hipSuccess)
-      return BSOFI_ERR_ALLOC;
-    if (hipMemcpyAsync(r.dfields, r.fields.data(), r.fields.size(), hipMemcpyHostToDevice, st) != hipSuccess)
-      return BSOFI_ERR_NO_DEVICE;
-    int rc = bsofi_dqmc_build(sl.ctx, r.dA, (int)N, r.n, r.p, s->lattices[c.lattice].d_expk, r.dfields, c.ep, c.em);
-    if (rc) return rc;
+    r.dA = (double*)pool_get(s, N * N * 8);
+    r.dtau = (double*)pool_get(s, N * 8);
+    r.dfields = (signed char*)pool_get(s, r.fields.size());
+    if (!r.dA || !r.dtau || !r.dfields) return BSOFI_ERR_ALLOC;
     mats[i].A = r.dA;
     mats[i].tau = r.dtau;
     mats[i].n = r.n;
     mats[i].p = r.p;
     mats[i].lda = (int)N;
   }
+  // The plan first (its creation waits for the slot's stream, which is idle
+  // now), then the inputs are built and the plan runs behind them.
   int rc = bsofi_plan_create(sl.ctx, (int)mats.size(), mats.data(), BSOFI_PHASE_ALL, nullptr, &sl.plan);
   if (rc) return rc;
+  for (size_t i = 0; i < batch.size(); ++i) {
+    Request& r = *batch[i];
+    const Chain& c = s->chains[r.chain];
+    if (hipMemcpyAsync(r.dfields, r.fields.data(), r.fields.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      return BSOFI_ERR_NO_DEVICE;
+    rc = bsofi_dqmc_build(sl.ctx, r.dA, r.n * r.p, r.n, r.p, s->lattices[c.lattice].d_expk, r.dfields, c.ep, c.em);
+    if (rc) return rc;
+  }
   rc = bsofi_plan_execute(sl.plan);
   if (rc) return rc;
   if (hipEventRecord(sl.done, st) != hipSuccess) return BSOFI_ERR_NO_DEVICE;
@@ -266,10 +311,8 @@ int bsofi_server_step(bsofi_server_t s, int* inflight) {
       r.state = DONE;
       r.info = info[i];
       r.t_done = t;
-      if (r.dfields) {
-        (void)hipFree(r.dfields);
-        r.dfields = nullptr;
-      }
+      pool_put(s, r.dfields, r.fields.size());
+      r.dfields = nullptr;
       s->completed++;
     }
     bsofi_plan_destroy(sl.plan);
@@ -294,6 +337,7 @@ int bsofi_server_step(bsofi_server_t s, int* inflight) {
       s->queue.pop_front();
     }
     if (batch.empty()) break;  // head does not fit: wait for memory
+    pool_trim(s, add);
     const double t = now_s();
     for (Request* r : batch) {
       r->state = RUNNING;
@@ -356,9 +400,11 @@ int bsofi_server_release(bsofi_server_t s, long long id) {
   if (it == s->reqs.end()) return -2;
   Request& r = *it->second;
   if (r.state != DONE) return 1;
-  (void)hipSetDevice(s->device);
-  if (r.dA) (void)hipFree(r.dA);
-  if (r.dtau) (void)hipFree(r.dtau);
+  const size_t N = (size_t)r.n * r.p;
+  pool_put(s, r.dA, N * N * 8);
+  pool_put(s, r.dtau, N * 8);
+  r.dA = nullptr;
+  r.dtau = nullptr;
   s->used -= r.result_bytes;
   s->reqs.erase(it);
   return 0;

@@ -168,6 +168,12 @@ int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
 int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti);
 int bsofi_ctx_get_tuning(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti);
 int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
+/* Destroyed plans return their device buffers (task tables, workspace) to
+ * the context, and later plans on it reuse them, so that creating and
+ * destroying plans in a loop allocates nothing once warm.  These report and
+ * free that cache (at most 8 buffers).  No counterpart in the reference. */
+size_t bsofi_ctx_cached_bytes(bsofi_ctx_t ctx);
+int bsofi_ctx_release_cache(bsofi_ctx_t ctx);
 /* Supported block sizes: 1 <= n <= BSOFI_MAX_N, L >= 2. */
 #define BSOFI_MAX_N 512
 int bsofi_check_args(int n, int L, int lda);
