@@ -46,6 +46,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(n, p):
+    """HBM bytes per scan launch from the committed rocprofv3 --pmc summary
+    for this (n, p) (FETCH_SIZE + WRITE_SIZE, scaled by a known-size copy),
+    or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"bsoi_scan_n{n}_p{p}_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 class DeviceSet:
     """K seeded DQMC matrices (and their 2p non-zero blocks) on the device."""
 
@@ -337,11 +352,27 @@ def main():
     ap.add_argument("--e2e", type=int, default=1, help="also time one inversion through the host-array H-API")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="only run y = -x over this many GiB of FP64 (3 launches): the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are out of scope "
               "(see the sharding design in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy > 0:
+        import torch
+
+        count = int(args.calibrate_copy * (1 << 30)) // 8
+        x = torch.ones(count, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": count * 8, "calibrate_copy_bytes_written": count * 8,
+                          "launches": 3}), flush=True)
+        return
 
     import bsofi_amd as B
     from bsofi_amd import flops
@@ -389,7 +420,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n, p),
+            "traffic_source": f"profiles/bsoi_scan_n{n}_p{p}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "flops_per_launch": bsoi_flops,
             "avg_launch_ms": round(r["scan_ms"], 4),
             "launches": r["scan_launches"],

@@ -87,6 +87,10 @@ def _load():
         "bsofi_ctx_stream": (_vp, [_vp]),
         "bsofi_ctx_synchronize": (i, [_vp]),
         "bsofi_ctx_set_tuning": (i, [_vp, i, i, i]),
+        "bsofi_plan_det": (i, [_vp, _dp, _dp]),
+        "bsofi_det": (i, [_vp, _vp, i, i, i, _vp, _dp, _dp]),
+        "bsofi_ctx_cached_bytes": (C.c_size_t, [_vp]),
+        "bsofi_ctx_release_cache": (i, [_vp]),
         "bsofi_ctx_get_tuning": (i, [_vp, i, C.POINTER(i), C.POINTER(i)]),
         "bsofi_ctx_clear_tuning": (i, [_vp]),
         "bsofi_workspace_bytes": (C.c_size_t, [i, i]),
@@ -325,6 +329,15 @@ class Plan:
             raise RuntimeError(f"bsofi_plan_info failed ({rc})")
         return list(out)[: self.count]
 
+    def det(self):
+        """[(sign(det A), log|det A|)] per matrix, from the last execution."""
+        n = max(self.count, 1)
+        sg, lg = (C.c_double * n)(), (C.c_double * n)()
+        rc = lib.bsofi_plan_det(self.h, sg, lg)
+        if rc:
+            raise RuntimeError(f"bsofi_plan_det failed ({rc})")
+        return [(sg[k], lg[k]) for k in range(self.count)]
+
     def time_family(self, family):
         lib.bsofi_plan_time_family(self.h, FAMILY[family] if isinstance(family, str) else family)
 
@@ -353,6 +366,15 @@ def invert_batched(ctx: Context, descs, phases=PHASE_ALL):
     if rc:
         raise RuntimeError(f"bsofi_invert_batched failed ({rc})")
     return list(info)[:count]
+
+
+def det(ctx: Context, dA_ptr, lda, n, p, dtau_ptr):
+    """(sign(det A), log|det A|) of a matrix factored by BSOF on the device."""
+    sg, lg = C.c_double(0), C.c_double(0)
+    rc = lib.bsofi_det(ctx.h, dA_ptr, lda, n, p, dtau_ptr, C.byref(sg), C.byref(lg))
+    if rc:
+        raise RuntimeError(f"bsofi_det failed ({rc})")
+    return sg.value, lg.value
 
 
 def pcyclic_residual(ctx: Context, blocks_ptr, G_ptr, ldg, n, p):

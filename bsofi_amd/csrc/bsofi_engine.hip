@@ -42,7 +42,8 @@ bool hip_ok(hipError_t e, const char* what) {
     if (!hip_ok((x), #x)) return BSOFI_ERR_NO_DEVICE; \
   } while (0)
 
-enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCAN = 5, F_LARFT = 6, F_FILL = 7 };
+enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCAN = 5, F_LARFT = 6, F_FILL = 7,
+              F_DET = 8 };
 
 struct Launch {
   int family = 0, variant = 0, grid = 0, ntasks = 0, ldcar = 0;
@@ -226,6 +227,8 @@ struct bsofi_plan_s {
   size_t ws_bytes = 0;
   size_t ws_block = 0;  // size of the owned workspace block
   int* d_info = nullptr;
+  double* d_det = nullptr;
+  bool has_det = false;
   int count = 0;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -245,6 +248,7 @@ struct MatJob {
   double* A;
   double* tau;
   int* info;
+  double* det;  // sign, log|det|
   double *T, *Q, *W, *Z1, *Z2, *Y;
 };
 
@@ -343,6 +347,18 @@ struct Planner {
       for (auto& kv : pt) emit(F_PANEL, kv.first, kv.second, pi[kv.first]);
       emit(F_REFL, refl_variant(rt), rt, ri);
     }
+  }
+
+  // sign(det A) and log|det A| from R's diagonal and tau, before BSTRI
+  // overwrites the diagonal blocks.
+  void det(std::vector<MatJob>& mats) {
+    std::vector<DetTask> dt;
+    std::vector<Item> di;
+    for (auto& M : mats) {
+      di.push_back({(int)dt.size(), 0});
+      dt.push_back({M.A, M.tau, M.det, M.g.lda, M.g.N});
+    }
+    emit(F_DET, 0, dt, di);
   }
 
   // -- BSTRI: src/bsofi_cpu.c:146-177, restructured by block rows ---------
@@ -643,6 +659,9 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     case F_EYE:
       hipLaunchKernelGGL(eye_kernel, dim3(L.grid), dim3(256), 0, s, (const EyeTask*)(base + L.task_off), L.ntasks);
       break;
+    case F_DET:
+      hipLaunchKernelGGL(det_kernel, dim3(L.grid), dim3(256), 0, s, (const DetTask*)(base + L.task_off));
+      break;
     case F_LARFT:
       hipLaunchKernelGGL(larft_kernel, dim3(L.grid), dim3(256), 0, s, (const PanelTask*)(base + L.task_off),
                          (const Item*)(base + L.item_off));
@@ -782,8 +801,14 @@ size_t bsofi_workspace_bytes(int n, int p) {
   return ws_layout(geometry(n, p, n * p), BSOFI_PHASE_ALL).total;
 }
 
+// Plan workspace: info words, then (sign, log|det|) per matrix, then the
+// matrices' own pieces.
+static size_t plan_header_bytes(int count) {
+  return al256((size_t)count * sizeof(int)) + al256((size_t)count * 2 * sizeof(double));
+}
+
 size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int phases) {
-  size_t total = al256((size_t)count * sizeof(int));
+  size_t total = plan_header_bytes(count);
   for (int i = 0; i < count; ++i) total += ws_layout(geometry(mats[i].n, mats[i].p, mats[i].lda), phases).total;
   return total;
 }
@@ -816,14 +841,17 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     P->own_ws = true;
   }
   P->d_info = (int*)P->ws;
+  P->d_det = (double*)(P->ws + al256((size_t)count * sizeof(int)));
+  P->has_det = (phases & BSOFI_PHASE_BSOF) != 0;
   std::vector<MatJob> jobs(count);
-  size_t off = al256((size_t)count * sizeof(int));
+  size_t off = plan_header_bytes(count);
   for (int i = 0; i < count; ++i) {
     MatJob& J = jobs[i];
     J.g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
     J.A = mats[i].A;
     J.tau = mats[i].tau;
     J.info = P->d_info + i;
+    J.det = P->d_det + 2 * i;
     const WsLayout L = ws_layout(J.g, phases);
     char* b = P->ws + off;
     J.T = (double*)(b + L.T);
@@ -846,7 +874,10 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     L.grid = 1;
     P->launches.push_back(L);
   }
-  if (phases & BSOFI_PHASE_BSOF) pl.bsof(jobs);
+  if (phases & BSOFI_PHASE_BSOF) {
+    pl.bsof(jobs);
+    pl.det(jobs);
+  }
   if (phases & BSOFI_PHASE_BSTRI) pl.bstri(jobs);
   if (phases & BSOFI_PHASE_BSOI) pl.bsoi(jobs, max_shmem);
   for (auto& L : P->launches) {
@@ -937,6 +968,47 @@ int bsofi_plan_info(bsofi_plan_t P, int* info) {
   HIPCHK(hipStreamSynchronize(P->ctx->stream));
   for (int i = 0; i < P->count; ++i)
     if (info[i] == INT_MAX) info[i] = 0;
+  return 0;
+}
+
+int bsofi_plan_det(bsofi_plan_t P, double* sign, double* logabs) {
+  if (!P || !sign || !logabs) return -1;
+  if (!P->has_det) return -2;
+  if (P->count == 0) return 0;
+  HIPCHK(hipSetDevice(P->ctx->device));
+  std::vector<double> h(2 * (size_t)P->count);
+  HIPCHK(hipMemcpyAsync(h.data(), P->d_det, h.size() * sizeof(double), hipMemcpyDeviceToHost, P->ctx->stream));
+  HIPCHK(hipStreamSynchronize(P->ctx->stream));
+  for (int i = 0; i < P->count; ++i) {
+    sign[i] = h[2 * i];
+    logabs[i] = h[2 * i + 1];
+  }
+  return 0;
+}
+
+int bsofi_det(bsofi_ctx_t ctx, const double* dA, int lda, int n, int p, const double* dtau, double* sign,
+              double* logabs) {
+  if (!ctx || !dA || !dtau || !sign || !logabs) return -1;
+  const int a = bsofi_check_args(n, p, lda);
+  if (a) return a;
+  HIPCHK(hipSetDevice(ctx->device));
+  // the task record and the result share one cached device buffer
+  size_t bytes = 256 + sizeof(DetTask);
+  char* buf = ctx_alloc(ctx, bytes);
+  if (!buf) return BSOFI_ERR_ALLOC;
+  DetTask t{dA, dtau, (double*)buf, lda, n * p};
+  double h[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(buf + 256, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(det_kernel, dim3(1), dim3(256), 0, ctx->stream, (const DetTask*)(buf + 256));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h, buf, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx_free(ctx, buf, bytes);
+  if (!hip_ok(e, "det")) return BSOFI_ERR_NO_DEVICE;
+  *sign = h[0];
+  *logabs = h[1];
   return 0;
 }
 

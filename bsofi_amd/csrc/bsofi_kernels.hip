@@ -613,6 +613,45 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
   build_T<32>(G, taus, nbj, t.T, t.ldt, &Gp[0][0], &Gp[1][0]);
 }
 
+// ---------------------------------------------------------------------------
+// Determinant from the factorization A = Q R of BSOF (one workgroup per
+// matrix).  Every Householder reflector with tau != 0 has determinant -1
+// (tau v^T v = 2 for dlarfg's reflectors), one with tau = 0 is the
+// identity, so
+//   sign(det A) = (-1)^#{tau_i != 0} prod sign(R_ii),
+//   log|det A|  = sum log|R_ii|.
+// A zero R_ii gives sign 0 and log -inf.  Fixed reduction order, so the
+// result is the same bits in a batch and alone.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void det_kernel(const DetTask* __restrict__ tasks) {
+  const DetTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x;
+  __shared__ double ls[256];
+  __shared__ int par[256];
+  double lg = 0.0;
+  int flips = 0, zero = 0;
+  for (int i = tid; i < t.N; i += 256) {
+    const double r = t.A[(size_t)i * t.lda + i];
+    zero |= (r == 0.0);
+    flips ^= (r < 0.0) ^ (t.tau[i] != 0.0);
+    lg += log(fabs(r));
+  }
+  ls[tid] = lg;
+  par[tid] = flips | (zero << 1);
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (tid < h) {
+      ls[tid] += ls[tid + h];
+      par[tid] = ((par[tid] ^ par[tid + h]) & 1) | ((par[tid] | par[tid + h]) & 2);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    t.out[0] = (par[0] & 2) ? 0.0 : ((par[0] & 1) ? -1.0 : 1.0);
+    t.out[1] = ls[0];
+  }
+}
+
 __global__ void fill_int_kernel(int* __restrict__ p, int count, int value) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < count) p[e] = value;

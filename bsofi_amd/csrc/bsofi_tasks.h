@@ -78,6 +78,15 @@ struct EyeTask {
   int dim, count;
 };
 
+// sign(det A) and log|det A| of one factored matrix (after BSOF): R's
+// diagonal A[i + i lda] and tau[i], i < N; out[0] = sign, out[1] = log|det|.
+struct DetTask {
+  const double* A;
+  const double* tau;
+  double* out;
+  int lda, N;
+};
+
 // Work-item record: which task a workgroup serves and which piece of it.
 struct Item {
   int task;

@@ -192,6 +192,16 @@ int bsofi_plan_use_graph(bsofi_plan_t plan, int on);
 int bsofi_plan_num_launches(bsofi_plan_t plan);
 /* Per-matrix status after execution (host copy, count ints). */
 int bsofi_plan_info(bsofi_plan_t plan, int* info);
+/* sign(det A) (+1, -1, or 0 if R is singular) and log|det A| of every matrix
+ * of a plan that ran BSOF, taken from R's diagonal and tau right after BSOF
+ * (before BSTRI overwrites R): each reflector with tau != 0 contributes
+ * det = -1.  Host arrays of the plan's count.  Returns -2 for a plan without
+ * the BSOF phase.  (The reference's examples/ftest_cpu/ftest_cpu.f90:134-138
+ * flips its sign on tau == 0 instead, i.e. it yields (-1)^N sign(det A).) */
+int bsofi_plan_det(bsofi_plan_t plan, double* sign, double* logabs);
+/* The same for one matrix already factored by BSOF on the device. */
+int bsofi_det(bsofi_ctx_t ctx, const double* dA, int lda, int n, int p, const double* dtau, double* sign,
+              double* logabs);
 /* Time every launch of one kernel family with HIP events on the plan's
  * stream; read the accumulated milliseconds and launch count after a sync.
  * family: 0 panel QR, 1 reflector apply, 2 tile GEMM, 3 diag TRTRI,

@@ -248,3 +248,46 @@ def test_tuned_variants_match_oracle(B, eps):
     ctx.clear_tuning()
     assert ctx.get_tuning(64) == (32, 2)
     ctx.close()
+
+
+def test_det_matches_slogdet(B):
+    """sign(det A) and log|det A| from the device factorization: against
+    numpy's LU-based slogdet, against the reference's tau/diag(R) loop
+    (which yields (-1)^N sign(det A)), through bsofi_det on a BSOF-only
+    plan, and the same bits in a batch as alone."""
+    import torch
+
+    cases = [dqmc.dqmc_matrix(4, 6, instance=31)[0], dqmc.dqmc_matrix(3, 5, instance=32)[0],
+             dqmc.random_pcyclic(7, 4, seed=5, identity_diag=False), dqmc.random_pcyclic(12, 3, seed=6)]
+    dims = [(16, 6), (9, 5), (7, 4), (12, 3)]
+    O = oracle.Oracle("auto", threads=1)
+    ctx = B.Context(0)
+    devs = [to_dev(A) for A in cases]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in dims]
+    torch.cuda.synchronize()
+    descs = [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t, (n, p) in zip(devs, taus, dims)]
+    plan = B.Plan(ctx, descs)
+    plan.execute()
+    batch = plan.det()
+    plan.close()
+    for k, (A0, (n, p)) in enumerate(zip(cases, dims)):
+        N = n * p
+        print(f"det case {k}: n={n} p={p}", flush=True)
+        s_ref, l_ref = np.linalg.slogdet(A0)
+        sg, lg = batch[k]
+        assert sg == s_ref
+        assert abs(lg - l_ref) <= 1e-10 * max(1.0, abs(l_ref))
+        Ao = A0.copy(order="F")
+        tau_o = np.zeros(N)
+        assert O.bsof(n, p, Ao, tau_o) == 0
+        assert sg == (-1) ** N * oracle.det_sign(tau_o, Ao)
+        # alone, BSOF only, then the standalone device call on the factor
+        d = to_dev(A0)
+        t = torch.zeros(N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        one = B.Plan(ctx, [(d.data_ptr(), t.data_ptr(), n, p, N)], phases=B.PHASE_BSOF)
+        one.execute()
+        assert one.det()[0] == batch[k]
+        one.close()
+        assert B.det(ctx, d.data_ptr(), N, n, p, t.data_ptr()) == batch[k]
+    ctx.close()
