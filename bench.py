@@ -314,8 +314,20 @@ def bench_e2e(B, lx, p, seed):
     return dict(seconds=dt, info=(i1, i2))
 
 
-def cpu_baseline(lx, p, seed, threads):
-    """The reference's CPU path (oracle/_ref) on the same matrix (instance 0)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(lx, p, seed, threads, reps=5):
+    """The reference's CPU path (oracle/_ref) on the same matrix (instance 0):
+    one warm-up inversion, then the median of `reps` (each on a fresh copy)."""
     import oracle
     from bsofi_amd import dqmc
 
@@ -327,14 +339,17 @@ def cpu_baseline(lx, p, seed, threads):
         kind = "port"
     A0, meta = dqmc.dqmc_matrix(lx, p, seed=seed, instance=0)
     n = meta["n"]
-    A = A0.copy(order="F")
-    t0 = time.perf_counter()
-    if kind == "reference":
-        tau, info = impl.invert(n, p, A)
-    else:
-        tau, info, _ = impl.invert(n, p, A)
-    dt = time.perf_counter() - t0
-    return dict(seconds=dt, kind=kind, info=info)
+    times, info = [], 0
+    for r in range(reps + 1):
+        A = A0.copy(order="F")
+        t0 = time.perf_counter()
+        if kind == "reference":
+            tau, info = impl.invert(n, p, A)
+        else:
+            tau, info, _ = impl.invert(n, p, A)
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    return dict(seconds=float(np.median(times)), kind=kind, info=info, reps=reps)
 
 
 def main():
@@ -495,7 +510,9 @@ def main():
             "unit": "inversions/s",
             "cores": threads,
             "kind": c["kind"],
-            "sample": f"one full inversion of the same n={n}, p={p} matrix (instance 0), MKL {threads} threads",
+            "sample": f"median of {c['reps']} full inversions (after one warm-up) of the same n={n}, p={p} "
+                      f"matrix (instance 0), MKL {threads} threads",
+            "cpu_model": cpu_model(),
             "gflops": round(tot_flops / c["seconds"] / 1e9, 2),
         }
     print(json.dumps(line), flush=True)
