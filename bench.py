@@ -107,10 +107,12 @@ class DeviceSet:
                                        self.p)
 
 
-def bench_plain(B, lx, p, steps, warmup, seed):
+def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0):
     import torch
 
     ctx = B.Context(0)
+    if nb or scan_mti:
+        ctx.set_tuning(lx * lx, nb, scan_mti)
     stream = torch.cuda.ExternalStream(ctx.stream)
     nbuf = steps + 1
     log(f"[bench] building {nbuf} device matrices n={lx * lx} p={p}")
@@ -367,6 +369,8 @@ def main():
     ap.add_argument("--e2e", type=int, default=1, help="also time one inversion through the host-array H-API")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--nb", type=int, default=0, help="sub-panel width for the plain run (0 = default)")
+    ap.add_argument("--scan-mti", type=int, default=0, help="scan tile height / 16 for the plain run (0 = default)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run y = -x over this many GiB of FP64 (3 launches): the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -396,7 +400,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed)
+    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed, args.nb, args.scan_mti)
     n, p, N = r["n"], r["p"], r["N"]
     value = args.steps / r["wall"]
     tot_flops = flops.bsofi(n, p)

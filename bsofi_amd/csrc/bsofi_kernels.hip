@@ -25,6 +25,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "bsofi_tasks.h"
 
 namespace bsofi {
@@ -235,6 +237,31 @@ __device__ __forceinline__ double scaled_norm_wave(const double (&x)[RPG], int l
   return amax * sqrt(wave_sum(ls, lane));
 }
 
+// sqrt and 1/x for arguments known to be normal, finite and positive (sqrt)
+// or nonzero (1/x) and far from the exponent limits, as in the panel's fast
+// path (|x| within [2^-960, 2^962]).  From the hardware approximations
+// v_rsq_f64 / v_rcp_f64, one Goldschmidt iteration and a Newton correction
+// for sqrt, two Newton iterations for 1/x: a few dependent FMAs, within an
+// ulp or two of the correctly rounded value, without the range scaling the
+// general library routines carry.
+__device__ __forceinline__ double sqrt_normal(double s) {
+  const double y = __builtin_amdgcn_rsq(s);
+  double g = s * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, s);
+  return fma(d, h, g);
+}
+
+__device__ __forceinline__ double rcp_normal(double x) {
+  double z = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, z, 1.0);
+  z = fma(z, e, z);
+  e = fma(-x, z, 1.0);
+  return fma(z, e, z);
+}
+
 struct Larfg {
   double tau, beta, scale;
 };
@@ -286,9 +313,17 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
   __shared__ double Tl[NB][NB + 1];
   __shared__ double Xl[NB][NB + 1];
 
+  double a[RPG][CPW];
+  // The column loop runs first on dlarfg's fast path alone, with no branch
+  // in it (a branch on the norm would stall every column on the norm's
+  // reduction); `bad` records whether any column left the fast path's range.
+  // Then, and only then, the panel -- still intact in memory, it is written
+  // back at the end -- is factored again with LAPACK's scaled path.
+  auto run = [&](auto safe_tag) -> bool {
+    constexpr bool SAFE = decltype(safe_tag)::value;
+    bool bad = false;
   for (int e = tid; e < NB * (NB + 1); e += 256) (&G[0][0])[e] = 0.0;
   if (tid < NB) taus[tid] = 0.0;
-  double a[RPG][CPW];
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
     const int row = lane + 64 * q;
@@ -331,9 +366,9 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
         // A sum of squares outside [2^-960, 2^960] (including x = 0) or
         // |alpha| >= 2^480 takes LAPACK's scaled path instead.
         const bool inrange = xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480;
-        double beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+        double beta = -copysign(sqrt_normal(fma(alpha, alpha, xn2)), alpha);
         const double amb = alpha - beta;
-        const double rr = 1.0 / (beta * amb);
+        const double rr = rcp_normal(beta * amb);
         double tau = -(amb * amb) * rr;
         double scale = beta * rr;
 
@@ -349,11 +384,15 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
         double tot[CPW];
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) tot[jj] = readlane64(P[0], wave_reduce_lane<CPW>(jj));
-        if (!inrange) {
-          const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
-          tau = lf.tau;
-          beta = lf.beta;
-          scale = lf.scale;
+        if constexpr (SAFE) {
+          if (!inrange) {
+            const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
+            tau = lf.tau;
+            beta = lf.beta;
+            scale = lf.scale;
+          }
+        } else {
+          bad |= !inrange;
         }
 
         // w_j = tau v_c^T a_j (j > c); G[j][c] = v_j^T v_c (j < c)
@@ -392,6 +431,12 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
         if (tid == 0) taus[c] = tau;
       }
     }
+  }
+    return bad;
+  };
+  if (run(std::false_type{})) {
+    __syncthreads();  // every wave is done with xs before it is refilled
+    run(std::true_type{});
   }
 
 #pragma unroll
