@@ -574,19 +574,18 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
     W[i][j] = (Wp[0][i][j] + Wp[1][i][j]) + (Wp[2][i][j] + Wp[3][i][j]);
   }
   __syncthreads();
-  for (int e = tid; e < 32 * 16; e += 256) {
-    const int i = e >> 4, j = e & 15;
-    double s2 = 0.0;
-    if (i < nbj) {
-      if (t.trans) {  // T^T: sum_{k <= i} T[k][i] W[k][j]
-#pragma unroll 4
-        for (int k = 0; k <= i; ++k) s2 = fma(Tl[k][i], W[k][j], s2);
-      } else {        // T: sum_{k >= i} T[i][k] W[k][j]
-#pragma unroll 4
-        for (int k = i; k < nbj; ++k) s2 = fma(Tl[i][k], W[k][j], s2);
-      }
+  // op(T) W on MFMA: waves 0 and 1 form rows 0-15 and 16-31 (K = 32).
+  // Tl is zero outside the nbj x nbj triangle, so no masking is needed.
+  if (wave < 2) {
+    const int i = wave * 16 + li;
+    d4 tw = d4zero();
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) {
+      const int k = 4 * kc + lk;
+      tw = mfma(t.trans ? Tl[k][i] : Tl[i][k], W[k][li], tw);
     }
-    Wf[i][j] = -s2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
   }
   __syncthreads();
   // phase 3: C += V Wf, then back to the row-major-lanes layout and out
