@@ -107,9 +107,10 @@ class DeviceSet:
                                        self.p)
 
 
-def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0):
+def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None):
     import torch
 
+    phases = B.PHASE_ALL if phases is None else phases
     ctx = B.Context(0)
     if nb or scan_mti:
         ctx.set_tuning(lx * lx, nb, scan_mti)
@@ -117,7 +118,7 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0):
     nbuf = steps + 1
     log(f"[bench] building {nbuf} device matrices n={lx * lx} p={p}")
     ds = DeviceSet(B, ctx, lx, p, nbuf, seed)
-    plans = [B.Plan(ctx, [ds.desc(k)]) for k in range(nbuf)]
+    plans = [B.Plan(ctx, [ds.desc(k)], phases=phases) for k in range(nbuf)]
     log(f"[bench] plan: {plans[0].num_launches} launches per inversion")
     # warm-up on buffer `steps` (rebuilt between warm-up steps, outside timing)
     for w in range(warmup):
@@ -146,7 +147,9 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0):
         scan_ms += ms
         scan_n += cnt
     infos = [pl.info()[0] for pl in plans[:steps]]
-    res = [ds.residual(k) for k in range(steps)]
+    # the residual needs the whole G; diagonal-only runs are checked by the
+    # tests (bit-equal to the full inverse's diagonal blocks)
+    res = [ds.residual(k) for k in range(steps)] if phases == B.PHASE_ALL else [0.0]
     for pl in plans:
         pl.close()
     out = dict(n=ds.n, p=p, N=ds.N, wall=wall, lat=lat, scan_ms=scan_ms / max(scan_n, 1), scan_launches=scan_n,
@@ -367,6 +370,7 @@ def main():
     ap.add_argument("--serve", type=int, default=1, help="run the serving loop")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--e2e", type=int, default=1, help="also time one inversion through the host-array H-API")
+    ap.add_argument("--diag", type=int, default=1, help="also time the diagonal-blocks-only mode on the same size")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--nb", type=int, default=0, help="sub-panel width for the plain run (0 = default)")
@@ -461,6 +465,15 @@ def main():
             "speedup_vs_plain": round(b_ips / plain_ips, 2),
             "resid_max_sampled": b["resid"],
             "info_nonzero": b["info_bad"],
+        }
+    if args.diag:
+        dg = bench_plain(B, args.lx, args.p, 3, 1, args.seed, phases=B.PHASE_ALL | B.PHASE_DIAG_ONLY)
+        line["diag_only"] = {
+            "workload": f"same n={n}, p={p} matrices, only the diagonal blocks G_ll written (DQMC equal-time "
+                        "Green's functions), BSOFI_PHASE_DIAG_ONLY",
+            "inversions_per_s": round(3 / dg["wall"], 3),
+            "ms_per_step": round(1e3 * dg["wall"] / 3, 3),
+            "scan_ms": round(dg["scan_ms"], 3),
         }
     if args.uneven:
         u = bench_uneven(B, args.uneven, args.seed, 2, 24, log)

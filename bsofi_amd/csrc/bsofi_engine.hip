@@ -484,7 +484,7 @@ struct Planner {
   }
 
   // -- BSOI: src/bsofi_cpu.c:283-333 ----------------------------------------
-  void bsoi(std::vector<MatJob>& mats, int max_shmem) {
+  void bsoi(std::vector<MatJob>& mats, int max_shmem, bool diag_only) {
     // (1) T factors from (V, tau), (2) Q_k = I, (3) Q_k <- B_j Q_k for
     // j = last .. 0 (dorgqr's backward accumulation), (4) row scans.
     std::vector<PanelTask> lt;
@@ -565,7 +565,7 @@ struct Planner {
         if (g.scan_mti != key.first || g.scan_ntw != key.second) continue;
         const int MT = 16 * g.scan_mti;
         const int id = (int)st.size();
-        st.push_back({M.A, M.Q, (long long)4 * g.n * g.n, g.lda, g.n, g.p, 0, g.N, 0});
+        st.push_back({M.A, M.Q, (long long)4 * g.n * g.n, g.lda, g.n, g.p, 0, g.N, diag_only ? 1 : 0});
         maxn = std::max(maxn, g.n);
         const int tiles = (g.N + MT - 1) / MT;
         for (int s = 0; s < tiles; ++s) {
@@ -879,7 +879,7 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     pl.det(jobs);
   }
   if (phases & BSOFI_PHASE_BSTRI) pl.bstri(jobs);
-  if (phases & BSOFI_PHASE_BSOI) pl.bsoi(jobs, max_shmem);
+  if (phases & BSOFI_PHASE_BSOI) pl.bsoi(jobs, max_shmem, (phases & BSOFI_PHASE_DIAG_ONLY) != 0);
   for (auto& L : P->launches) {
     if (L.family == F_SCAN) {
       const int variant = L.variant;

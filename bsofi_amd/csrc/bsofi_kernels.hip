@@ -848,6 +848,9 @@ __global__ void eye_kernel(const EyeTask* __restrict__ tasks, int ntasks) {
 // is read through an upper-triangular mask because A still holds the
 // reflectors below its diagonal.  The 2n output columns are split in
 // 16-column tiles dealt round-robin to the 4 waves (NTW tiles per wave).
+// With diag_only the workgroup stops once its rows' diagonal blocks are
+// done (block row r is finished at k = r - 1) and writes only G_rr; the
+// diagonal blocks are the same bits as in the full inverse.
 template <int MTI, int NTW>
 __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restrict__ tasks,
                                                         const Item* __restrict__ items, int ldcar) {
@@ -884,10 +887,12 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   __syncthreads();
 
   const int rbmin = R0 / n;
+  const bool diag = t.diag_only != 0;
+  const int kstop = diag ? max(rbmin - 1, 0) : 0;
   int bcol[NTW];
 #pragma unroll
   for (int b = 0; b < NTW; ++b) bcol[b] = (wave + 4 * b) * 16 + li;
-  for (int k = p - 2; k >= 0; --k) {
+  for (int k = p - 2; k >= kstop; --k) {
     const double* __restrict__ Qk = t.Q + (size_t)k * t.qstride;
     d4 acc[MTI][NTW];
 #pragma unroll
@@ -971,17 +976,21 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
             carry[col * ldcar + i] = acc[a][b][r];
           } else {
             const int row = R0 + i;
-            if (row < N) A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[a][b][r];
+            if (row < N && (!diag || row / n == k + 1))
+              A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[a][b][r];
           }
         }
       }
     }
     __syncthreads();
   }
-  for (int e = tid; e < n * MT; e += 256) {
-    const int i = e % MT, kk = e / MT;
-    const int row = R0 + i;
-    if (row < N) A[(size_t)kk * lda + row] = carry[kk * ldcar + i];
+  // block column 0 (the carry after k = 0), for diag_only only its diagonal
+  if (!diag || rbmin == 0) {
+    for (int e = tid; e < n * MT; e += 256) {
+      const int i = e % MT, kk = e / MT;
+      const int row = R0 + i;
+      if (row < N && (!diag || row < n)) A[(size_t)kk * lda + row] = carry[kk * ldcar + i];
+    }
   }
 }
 

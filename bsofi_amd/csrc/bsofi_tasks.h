@@ -68,7 +68,8 @@ struct ScanTask {
   double* A;
   const double* Q;
   long long qstride;
-  int lda, n, p, r0, rows, pad;
+  int lda, n, p, r0, rows;
+  int diag_only;  // 1: produce only the diagonal blocks G_ll
 };
 
 // Identity fill of count square matrices of order dim, stride apart.

@@ -150,6 +150,12 @@ typedef struct {
 #define BSOFI_PHASE_BSTRI 2
 #define BSOFI_PHASE_BSOI 4
 #define BSOFI_PHASE_ALL 7
+/* With BSOFI_PHASE_BSOI: produce only the diagonal blocks G_ll (the
+ * equal-time Green's functions of DQMC), in place in the diagonal blocks of
+ * A; the other blocks of A are left holding intermediate values.  The
+ * diagonal blocks are bit-identical to those of the full inverse.  No
+ * counterpart in the reference. */
+#define BSOFI_PHASE_DIAG_ONLY 8
 
 /* Contexts.  stream = NULL makes the context create its own stream. */
 int bsofi_device_count(void);

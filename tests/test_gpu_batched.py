@@ -291,3 +291,43 @@ def test_det_matches_slogdet(B):
         one.close()
         assert B.det(ctx, d.data_ptr(), N, n, p, t.data_ptr()) == batch[k]
     ctx.close()
+
+
+def test_diag_only_blocks_equal_full_inverse(B):
+    """BSOFI_PHASE_DIAG_ONLY: the diagonal blocks G_ll equal, bit for bit,
+    those of the full inverse -- for block sizes whose scan tiles straddle
+    block rows (n = 9, 36, 100), every scan tile height, and in a batch."""
+    import torch
+
+    cases = [(3, 5), (6, 4), (10, 3), (4, 7)]
+    ctx = B.Context(0)
+    for mti in (0, 1, 2, 4):
+        mats = []
+        for k, (lx, p) in enumerate(cases):
+            n = lx * lx
+            if mti:
+                try:
+                    ctx.set_tuning(n, 0, mti)
+                except ValueError:
+                    ctx.set_tuning(n, 0, 0)
+            A0, _ = dqmc.dqmc_matrix(lx, p, instance=90 + k)
+            mats.append((A0, n, p))
+        full = [to_dev(A0) for A0, _, _ in mats]
+        diag = [to_dev(A0) for A0, _, _ in mats]
+        tf = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for _, n, p in mats]
+        td = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for _, n, p in mats]
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)
+                                      for d, t, (_, n, p) in zip(full, tf, mats)]) == [0] * len(mats)
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)
+                                      for d, t, (_, n, p) in zip(diag, td, mats)],
+                                phases=B.PHASE_ALL | B.PHASE_DIAG_ONLY) == [0] * len(mats)
+        for k, ((A0, n, p), f, d) in enumerate(zip(mats, full, diag)):
+            print(f"diag-only case n={n} p={p} scan_mti={mti or 'default'}", flush=True)
+            G = from_dev(f, n * p)
+            D = from_dev(d, n * p)
+            for l in range(p):
+                s = slice(l * n, (l + 1) * n)
+                assert np.array_equal(D[s, s], G[s, s]), (k, l)
+        ctx.clear_tuning()
+    ctx.close()
