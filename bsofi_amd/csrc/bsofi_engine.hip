@@ -362,13 +362,30 @@ struct Planner {
   }
 
   // -- BSTRI: src/bsofi_cpu.c:146-177, restructured by block rows ---------
-  void gemm_items(std::vector<GemmTask>& tasks, std::vector<Item>& items, GemmTask t) {
-    if (t.M <= 0 || t.N <= 0) return;
-    t.tiles_n = (t.N + 63) / 64;
-    const int tm = (t.M + 63) / 64;
-    const int id = (int)tasks.size();
-    tasks.push_back(t);
-    for (int s = 0; s < tm * t.tiles_n; ++s) items.push_back({id, s});
+  // GEMM tasks of one launch step: 128 x 128 tiles where a product still
+  // yields >= 192 of them (enough to fill the GPU), 64 x 64 tiles otherwise.
+  static bool gemm_big(int M, int N) { return (long long)(M / 128) * (N / 128) >= 192; }
+
+  struct GemmBatch {
+    std::vector<GemmTask> t64, t128;
+    std::vector<Item> i64, i128;
+    void add(GemmTask t) {
+      if (t.M <= 0 || t.N <= 0) return;
+      const bool big = gemm_big(t.M, t.N);
+      const int tile = big ? 128 : 64;
+      auto& tasks = big ? t128 : t64;
+      auto& items = big ? i128 : i64;
+      t.tiles_n = (t.N + tile - 1) / tile;
+      const int tm = (t.M + tile - 1) / tile;
+      const int id = (int)tasks.size();
+      tasks.push_back(t);
+      for (int s = 0; s < tm * t.tiles_n; ++s) items.push_back({id, s});
+    }
+  };
+
+  void emit_gemm(const GemmBatch& b) {
+    emit(F_GEMM, 0, b.t64, b.i64);
+    emit(F_GEMM, 1, b.t128, b.i128);
   }
 
   static GemmTask gt(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
@@ -419,8 +436,7 @@ struct Planner {
       out.push_back(L);
     }
     for (int J = 1; J < maxJ; ++J) {
-      std::vector<GemmTask> t1, t2;
-      std::vector<Item> i1, i2;
+      GemmBatch b1, b2;
       for (auto& M : mats) {
         const Geo& g = M.g;
         if (32 * J >= g.n) continue;
@@ -428,44 +444,39 @@ struct Planner {
         for (int i = 0; i < g.p; ++i) {
           double* Bii = blk(M, i, i);
           double* Yi = M.Y + (size_t)i * g.n * 32;
-          gemm_items(t1, i1, gt(Bii + (size_t)32 * J * g.lda, g.lda, Bii + (size_t)32 * J * (g.lda + 1), g.lda, Yi,
-                                g.n, 32 * J, nbJ, nbJ, 1.0, 0, 1));
-          gemm_items(t2, i2, gt(Bii, g.lda, Yi, g.n, Bii + (size_t)32 * J * g.lda, g.lda, 32 * J, nbJ, 32 * J, -1.0, 1,
-                                0));
+          b1.add(gt(Bii + (size_t)32 * J * g.lda, g.lda, Bii + (size_t)32 * J * (g.lda + 1), g.lda, Yi, g.n, 32 * J,
+                    nbJ, nbJ, 1.0, 0, 1));
+          b2.add(gt(Bii, g.lda, Yi, g.n, Bii + (size_t)32 * J * g.lda, g.lda, 32 * J, nbJ, 32 * J, -1.0, 1, 0));
         }
       }
-      emit(F_GEMM, 0, t1, i1);
-      emit(F_GEMM, 0, t2, i2);
+      emit_gemm(b1);
+      emit_gemm(b2);
     }
     // (2) W_i = -X_ii R_{i,i+1} (i <= p-2) and Z1_i = R_{i,p-1} X_{p-1,p-1}
     // (i <= p-3), then Z2_i = -X_ii Z1_i: the two TRMMs of
     // src/bsofi_cpu.c:160,163,165 applied to every row at once.
     {
-      std::vector<GemmTask> t1, t2;
-      std::vector<Item> i1, i2;
+      GemmBatch b1, b2;
       for (auto& M : mats) {
         const Geo& g = M.g;
         const size_t n2 = (size_t)g.n * g.n;
         for (int i = 0; i <= g.p - 2; ++i)
-          gemm_items(t1, i1, gt(blk(M, i, i), g.lda, blk(M, i, i + 1), g.lda, M.W + i * n2, g.n, g.n, g.n, g.n, -1.0, 1,
-                                0));
+          b1.add(gt(blk(M, i, i), g.lda, blk(M, i, i + 1), g.lda, M.W + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0));
         for (int i = 0; i <= g.p - 3; ++i) {
-          gemm_items(t1, i1, gt(blk(M, i, g.p - 1), g.lda, blk(M, g.p - 1, g.p - 1), g.lda, M.Z1 + i * n2, g.n, g.n,
-                                g.n, g.n, 1.0, 0, 1));
-          gemm_items(t2, i2, gt(blk(M, i, i), g.lda, M.Z1 + i * n2, g.n, M.Z2 + i * n2, g.n, g.n, g.n, g.n, -1.0, 1,
-                                0));
+          b1.add(gt(blk(M, i, g.p - 1), g.lda, blk(M, g.p - 1, g.p - 1), g.lda, M.Z1 + i * n2, g.n, g.n, g.n, g.n,
+                    1.0, 0, 1));
+          b2.add(gt(blk(M, i, i), g.lda, M.Z1 + i * n2, g.n, M.Z2 + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0));
         }
       }
-      emit(F_GEMM, 0, t1, i1);
-      emit(F_GEMM, 0, t2, i2);
+      emit_gemm(b1);
+      emit_gemm(b2);
     }
     // (3) Row sweep, bottom-up: X_{i,i+1:p-1} = W_i X_{i+1,i+1:p-1}, plus
     // Z2_i on the last block column (src/bsofi_cpu.c:161-167).
     int rounds = 0;
     for (auto& M : mats) rounds = std::max(rounds, M.g.p - 1);
     for (int s = 0; s < rounds; ++s) {
-      std::vector<GemmTask> t;
-      std::vector<Item> it;
+      GemmBatch b;
       for (auto& M : mats) {
         const Geo& g = M.g;
         const int i = g.p - 2 - s;
@@ -477,9 +488,9 @@ struct Planner {
           q.ldd = g.n;
           q.dcol0 = (g.p - 2 - i) * g.n;
         }
-        gemm_items(t, it, q);
+        b.add(q);
       }
-      emit(F_GEMM, 0, t, it);
+      emit_gemm(b);
     }
   }
 
@@ -650,8 +661,12 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_GEMM:
-      hipLaunchKernelGGL(gemm_kernel, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
-                         (const Item*)(base + L.item_off));
+      if (L.variant == 1)
+        hipLaunchKernelGGL(gemm_kernel<4>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
+                           (const Item*)(base + L.item_off));
+      else
+        hipLaunchKernelGGL(gemm_kernel<2>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
+                           (const Item*)(base + L.item_off));
       break;
     case F_TRTRI:
       hipLaunchKernelGGL(trtri_diag_kernel, dim3(L.grid), dim3(64), 0, s, (const TrtriTask*)(base + L.task_off));
@@ -1109,22 +1124,31 @@ int bsofi_debug_gemm(bsofi_ctx_t ctx, int M, int N, int K, double alpha, const d
   t.amask = amask;
   t.bmask = bmask;
   t.dcol0 = INT_MAX;
-  t.tiles_n = (N + 63) / 64;
-  const int tiles = ((M + 63) / 64) * t.tiles_n;
+  // the same kernel choice as the planner's GemmBatch
+  const bool big = Planner::gemm_big(M, N);
+  const int tile = big ? 128 : 64;
+  t.tiles_n = (N + tile - 1) / tile;
+  const int tiles = ((M + tile - 1) / tile) * t.tiles_n;
   if (tiles == 0) return 0;
   std::vector<Item> items(tiles);
   for (int s2 = 0; s2 < tiles; ++s2) items[s2] = {0, s2};
-  char* d = nullptr;
   const size_t ib = al256(sizeof(GemmTask));
-  HIPCHK(hipMalloc((void**)&d, ib + items.size() * sizeof(Item)));
+  size_t bytes = ib + items.size() * sizeof(Item);
+  char* d = ctx_alloc(ctx, bytes);
+  if (!d) return BSOFI_ERR_ALLOC;
   hipError_t e = hipMemcpyAsync(d, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(d + ib, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(gemm_kernel, dim3(tiles), dim3(256), 0, ctx->stream, (const GemmTask*)d, (const Item*)(d + ib));
+    if (big)
+      hipLaunchKernelGGL(gemm_kernel<4>, dim3(tiles), dim3(256), 0, ctx->stream, (const GemmTask*)d,
+                         (const Item*)(d + ib));
+    else
+      hipLaunchKernelGGL(gemm_kernel<2>, dim3(tiles), dim3(256), 0, ctx->stream, (const GemmTask*)d,
+                         (const Item*)(d + ib));
     e = hipStreamSynchronize(ctx->stream);
   }
-  (void)hipFree(d);
+  ctx_free(ctx, d, bytes);
   return hip_ok(e, "debug_gemm") ? 0 : BSOFI_ERR_NO_DEVICE;
 }
 

@@ -702,72 +702,91 @@ __global__ void fill_int_kernel(int* __restrict__ p, int count, int value) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile GEMM (K4): 64 x 64 output tile per workgroup, K staged through LDS in
-// chunks of 16, 2 x 2 waves of 32 x 32 (2 x 2 MFMA tiles each).
+// Tile GEMM (K4): C = alpha op(A) op(B) + beta C (+ D), 32 WT x 32 WT output
+// tile per workgroup (WT = 2: 64 x 64, WT = 4: 128 x 128), each wave a
+// 16 WT x 16 WT quarter (WT x WT MFMA tiles).  K runs in chunks of 16
+// through two LDS buffers: the global loads of chunk c + 1 are in flight
+// while chunk c is multiplied, one barrier per chunk.  Global reads are
+// unit-stride across threads (rows of A, k of B).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void gemm_kernel(const GemmTask* __restrict__ tasks,
+template <int WT>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict__ tasks,
                                                    const Item* __restrict__ items) {
+  constexpr int TM = 32 * WT;          // tile edge
+  constexpr int AK = 256 / TM;         // k rows of A per pass of the block
+  constexpr int NA = 16 / AK;          // A values per thread per chunk
+  constexpr int NBV = TM / 16;         // B values per thread per chunk
   const Item it = items[blockIdx.x];
   const GemmTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int tm = it.sub / t.tiles_n, tn = it.sub % t.tiles_n;
-  const int i0 = tm * 64, j0 = tn * 64;
+  const int i0 = tm * TM, j0 = tn * TM;
   const int wi = wave & 1, wj = wave >> 1;
-
-  __shared__ double As[64][17];
-  __shared__ double Bs[64][17];
-
-  d4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4zero();
-
   const int M = t.M, N = t.N, K = t.K;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    // A tile: 64 rows x 16 k
+
+  __shared__ double As[2][TM][17];
+  __shared__ double Bs[2][TM][17];
+
+  const int ia = tid % TM, ka = tid / TM;
+  const int kb = tid & 15, jb = tid >> 4;
+  double ra[NA], rb[NBV];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = tid & 63, kk = (tid >> 6) + 4 * r;
-      const int gi = i0 + i, gk = k0 + kk;
-      double v = 0.0;
-      if (gi < M && gk < K && !(t.amask && gi > gk)) v = t.A[(size_t)gk * t.lda + gi];
-      As[i][kk] = v;
+    for (int r = 0; r < NA; ++r) {
+      const int gi = i0 + ia, gk = k0 + ka + AK * r;
+      ra[r] = (gi < M && gk < K && !(t.amask && gi > gk)) ? t.A[(size_t)gk * t.lda + gi] : 0.0;
     }
-    // B tile: 16 k x 64 cols
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int kk = tid & 15, j = (tid >> 4) + 16 * r;
-      const int gk = k0 + kk, gj = j0 + j;
-      double v = 0.0;
-      if (gk < K && gj < N && !(t.bmask && gk > gj)) v = t.B[(size_t)gj * t.ldb + gk];
-      Bs[j][kk] = v;
+    for (int r = 0; r < NBV; ++r) {
+      const int gk = k0 + kb, gj = j0 + jb + 16 * r;
+      rb[r] = (gk < K && gj < N && !(t.bmask && gk > gj)) ? t.B[(size_t)gj * t.ldb + gk] : 0.0;
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) As[buf][ia][ka + AK * r] = ra[r];
+#pragma unroll
+    for (int r = 0; r < NBV; ++r) Bs[buf][jb + 16 * r][kb] = rb[r];
+  };
+
+  d4 acc[WT][WT];
+#pragma unroll
+  for (int a = 0; a < WT; ++a)
+#pragma unroll
+    for (int b = 0; b < WT; ++b) acc[a][b] = d4zero();
+
+  const int nch = (K + 15) / 16;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nch) fetch(16 * (c + 1));
 #pragma unroll
     for (int ks = 0; ks < 16; ks += 4) {
-      double af[2], bf[2];
+      double af[WT], bf[WT];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[wi * 32 + a * 16 + li][ks + lk];
+      for (int a = 0; a < WT; ++a) af[a] = As[buf][wi * 16 * WT + a * 16 + li][ks + lk];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[wj * 32 + b * 16 + li][ks + lk];
+      for (int b = 0; b < WT; ++b) bf[b] = Bs[buf][wj * 16 * WT + b * 16 + li][ks + lk];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < WT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = mfma(af[a], bf[b], acc[a][b]);
+        for (int b = 0; b < WT; ++b) acc[a][b] = mfma(af[a], bf[b], acc[a][b]);
     }
+    if (c + 1 < nch) stash(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WT; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int col = j0 + wj * 32 + b * 16 + li;
+    for (int b = 0; b < WT; ++b) {
+      const int col = j0 + wj * 16 * WT + b * 16 + li;
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = i0 + wi * 32 + a * 16 + crow(lk, r);
+        const int row = i0 + wi * 16 * WT + a * 16 + crow(lk, r);
         if (row >= M) continue;
         double v = t.alpha * acc[a][b][r];
         double* cp = t.C + (size_t)col * t.ldc + row;

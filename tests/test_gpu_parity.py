@@ -225,8 +225,10 @@ def test_debug_gemm_matches_numpy(B):
 
     ctx = B.Context(0)
     rng = np.random.default_rng(1)
+    # the last one has >= 192 128 x 128 tiles (the big-tile kernel), the rest use 64 x 64 tiles
     for (M, N, K, am, bm) in [(16, 16, 4, 0, 0), (100, 70, 90, 0, 0), (70, 130, 70, 1, 0), (50, 50, 50, 0, 1),
-                              (5, 3, 2, 0, 0)]:
+                              (5, 3, 2, 0, 0), (128, 256, 16, 0, 0), (256, 300, 100, 0, 1), (200, 520, 37, 1, 0),
+                              (300, 1000, 256, 1, 1), (2048, 1664, 64, 0, 0)]:
         A = rng.standard_normal((M, K))
         Bm = rng.standard_normal((K, N))
         Cm = rng.standard_normal((M, N))
