@@ -46,6 +46,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def fp64_rates(B):
+    """This GPU's FP64 rates for back-to-back instructions of one kind on
+    every CU (bsofi_fp64_rate): the ceilings the kernels actually face."""
+    ctx = B.Context(0)
+    try:
+        return {k: round(B.fp64_rate(ctx, i), 1) for i, k in enumerate(("mfma_16x16x4", "mfma_4x4x4", "vector_fma"))}
+    finally:
+        ctx.close()
+
+
 def measured_traffic(n, p):
     """HBM bytes per scan launch from the committed rocprofv3 --pmc summary
     for this (n, p) (FETCH_SIZE + WRITE_SIZE, scaled by a known-size copy),
@@ -445,6 +455,7 @@ def main():
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
             "traffic": measured_traffic(n, p),
             "traffic_source": f"profiles/bsoi_scan_n{n}_p{p}_hbm_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "achievable_fp64_tflops": fp64_rates(B),
             "flops_per_launch": bsoi_flops,
             "avg_launch_ms": round(r["scan_ms"], 4),
             "launches": r["scan_launches"],

@@ -90,6 +90,7 @@ def _load():
         "bsofi_plan_det": (i, [_vp, _dp, _dp]),
         "bsofi_det": (i, [_vp, _vp, i, i, i, _vp, _dp, _dp]),
         "bsofi_ctx_cached_bytes": (C.c_size_t, [_vp]),
+        "bsofi_fp64_rate": (i, [_vp, i, _dp]),
         "bsofi_ctx_release_cache": (i, [_vp]),
         "bsofi_ctx_get_tuning": (i, [_vp, i, C.POINTER(i), C.POINTER(i)]),
         "bsofi_ctx_clear_tuning": (i, [_vp]),
@@ -375,6 +376,15 @@ def det(ctx: Context, dA_ptr, lda, n, p, dtau_ptr):
     if rc:
         raise RuntimeError(f"bsofi_det failed ({rc})")
     return sg.value, lg.value
+
+
+def fp64_rate(ctx: Context, kind):
+    """Achievable FP64 TFLOP/s: kind 0 = MFMA 16x16x4, 1 = MFMA 4x4x4, 2 = FMA."""
+    out = C.c_double(0)
+    rc = lib.bsofi_fp64_rate(ctx.h, kind, C.byref(out))
+    if rc:
+        raise RuntimeError(f"bsofi_fp64_rate failed ({rc})")
+    return out.value
 
 
 def pcyclic_residual(ctx: Context, blocks_ptr, G_ptr, ldg, n, p):

@@ -1152,6 +1152,43 @@ int bsofi_debug_gemm(bsofi_ctx_t ctx, int M, int N, int K, double alpha, const d
   return hip_ok(e, "debug_gemm") ? 0 : BSOFI_ERR_NO_DEVICE;
 }
 
+int bsofi_fp64_rate(bsofi_ctx_t ctx, int kind, double* tflops) {
+  if (!ctx || !tflops || kind < 0 || kind > 2) return -1;
+  HIPCHK(hipSetDevice(ctx->device));
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int grid = 2 * cus;
+  const int iters = kind == 2 ? 40000 : (kind == 1 ? 40000 : 10000);
+  size_t bytes = (size_t)grid * 256 * sizeof(double);
+  double* out = (double*)ctx_alloc(ctx, bytes);
+  if (!out) return BSOFI_ERR_ALLOC;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+    (void)hipEventRecord(e0, ctx->stream);
+    if (kind == 0)
+      hipLaunchKernelGGL(fp64_rate_kernel<0>, dim3(grid), dim3(256), 0, ctx->stream, out, iters, 1.0, 0.5);
+    else if (kind == 1)
+      hipLaunchKernelGGL(fp64_rate_kernel<1>, dim3(grid), dim3(256), 0, ctx->stream, out, iters, 1.0, 0.5);
+    else
+      hipLaunchKernelGGL(fp64_rate_kernel<2>, dim3(grid), dim3(256), 0, ctx->stream, out, iters, 0.999, 1e-3);
+    (void)hipEventRecord(e1, ctx->stream);
+    e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (rep > 0) best = std::min(best, ms);  // the first launch warms up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  ctx_free(ctx, (char*)out, bytes);
+  if (!hip_ok(e, "fp64_rate")) return BSOFI_ERR_NO_DEVICE;
+  const double per_wave = kind == 0 ? 8.0 * 2048 : (kind == 1 ? 8.0 * 512 : 8.0 * 64 * 2);
+  *tflops = (double)grid * 4 * iters * per_wave / (best * 1e-3) / 1e12;
+  return 0;
+}
+
 int bsofi_dqmc_build(bsofi_ctx_t ctx, double* dA, int lda, int n, int p, const double* dexpk,
                      const signed char* dfields, double ep, double em) {
   if (!ctx) return -1;

@@ -1014,6 +1014,44 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
 }
 
 // ---------------------------------------------------------------------------
+// FP64 rate calibration: independent chains of one instruction kind on
+// every CU (KIND 0: v_mfma_f64_16x16x4f64, 1: v_mfma_f64_4x4x4f64 with its
+// 4 blocks, 2: v_fma_f64), 8 chains per wave; the result is stored so
+// nothing is dropped.  Flops per iteration per wave: 8 x 2048, 8 x 512,
+// 8 x 64 x 2.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(256) void fp64_rate_kernel(double* __restrict__ out, int iters, double a, double b) {
+  const double x = a + threadIdx.x * 1e-9;
+  double s = 0.0;
+  if constexpr (KIND == 0) {
+    d4 acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = d4zero();
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = mfma(x, b, acc[c]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += (acc[c][0] + acc[c][1]) + (acc[c][2] + acc[c][3]);
+  } else {
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = KIND == 1 ? 0.0 : x + c;
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if constexpr (KIND == 1)
+          acc[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(x, b, acc[c], 0, 0, 0);
+        else
+          acc[c] = fma(acc[c], a, b);
+      }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += acc[c];
+  }
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
 // DQMC matrix builder: A = I + corner B_1 - subdiagonal B_l, with
 // B_l[:, i] = expk[:, i] * (s_{l,i} > 0 ? ep : em).
 // ---------------------------------------------------------------------------

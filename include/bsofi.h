@@ -180,6 +180,11 @@ int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
  * free that cache (at most 8 buffers).  No counterpart in the reference. */
 size_t bsofi_ctx_cached_bytes(bsofi_ctx_t ctx);
 int bsofi_ctx_release_cache(bsofi_ctx_t ctx);
+/* Achievable FP64 rate of this GPU (TFLOP/s) for one instruction kind run
+ * back to back on every CU: 0 = v_mfma_f64_16x16x4f64 (what the kernels
+ * use), 1 = v_mfma_f64_4x4x4f64, 2 = v_fma_f64.  A calibration for roofline
+ * figures next to the datasheet peak.  No counterpart in the reference. */
+int bsofi_fp64_rate(bsofi_ctx_t ctx, int kind, double* tflops);
 /* Supported block sizes: 1 <= n <= BSOFI_MAX_N, L >= 2. */
 #define BSOFI_MAX_N 512
 int bsofi_check_args(int n, int L, int lda);

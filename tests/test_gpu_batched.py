@@ -331,3 +331,17 @@ def test_diag_only_blocks_equal_full_inverse(B):
                 assert np.array_equal(D[s, s], G[s, s]), (k, l)
         ctx.clear_tuning()
     ctx.close()
+
+
+def test_fp64_rate_calibration(B):
+    """bsofi_fp64_rate returns a plausible positive rate for every kind
+    (MI355X datasheet FP64 peak 78.6 TFLOP/s; a reading above twice that
+    would mean a miscounted kernel)."""
+    ctx = B.Context(0)
+    for kind in range(3):
+        r = B.fp64_rate(ctx, kind)
+        print(f"fp64 rate kind {kind}: {r:.1f} TFLOP/s", flush=True)
+        assert 1.0 < r < 160.0
+    with pytest.raises(RuntimeError):
+        B.fp64_rate(ctx, 3)
+    ctx.close()
