@@ -281,12 +281,17 @@ struct Planner {
     out.push_back(L);
   }
 
-  // refl_apply_kernel instance: row tiles per wave for the largest m
+  // refl_apply_kernel instance for the largest m: 4 waves with 1, 2 or 4
+  // row tiles each up to m = 256 (variant = RT), else 8 waves with 4 or 8
+  // (variant = 100 + RT).
   static int refl_variant(const std::vector<ReflTask>& rt) {
     int mm = 0;
     for (auto& q : rt) mm = std::max(mm, q.m);
-    const int tiles = (mm + 63) / 64;
-    return tiles <= 1 ? 1 : tiles <= 2 ? 2 : tiles <= 4 ? 4 : tiles <= 8 ? 8 : 16;
+    if (mm <= 256) {
+      const int tiles = (mm + 63) / 64;
+      return tiles <= 1 ? 1 : tiles <= 2 ? 2 : 4;
+    }
+    return mm <= 512 ? 104 : 108;
   }
 
   // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------
@@ -652,11 +657,11 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       auto t = (const ReflTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
       switch (L.variant) {
-        case 1: hipLaunchKernelGGL(refl_apply_kernel<1>, dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 2: hipLaunchKernelGGL(refl_apply_kernel<2>, dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 4: hipLaunchKernelGGL(refl_apply_kernel<4>, dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 8: hipLaunchKernelGGL(refl_apply_kernel<8>, dim3(L.grid), dim3(256), 0, s, t, it); break;
-        default: hipLaunchKernelGGL(refl_apply_kernel<16>, dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 1: hipLaunchKernelGGL((refl_apply_kernel<1, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 2: hipLaunchKernelGGL((refl_apply_kernel<2, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 4: hipLaunchKernelGGL((refl_apply_kernel<4, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+        case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it); break;
+        default: hipLaunchKernelGGL((refl_apply_kernel<8, 8>), dim3(L.grid), dim3(512), 0, s, t, it); break;
       }
       break;
     }

@@ -467,7 +467,9 @@ __device__ __forceinline__ double vload(const double* __restrict__ V, int ldv, i
 }
 
 // The target slab lives in MFMA accumulator registers for the whole kernel:
-// wave w owns row tiles w, w+4, ... (RT of them, m <= 64 RT).  The C layout
+// NW waves (4 or 8); wave w owns row tiles w, w+NW, ... (RT of them,
+// m <= 16 NW RT).  Eight waves halve the loads each lane has in flight at
+// entry (a wave keeps at most 63 outstanding) and put two waves per SIMD.  The C layout
 // of a 16 x 16 accumulator tile is exactly four consecutive B fragments of
 // a K = 16 block (register r = rows 4r..4r+3), so phase 1 multiplies
 // straight out of the accumulators.
@@ -477,19 +479,20 @@ __device__ __forceinline__ double vload(const double* __restrict__ V, int ldv, i
 // V[rt*16 + li][4 kq + lk] (phase 3's A fragments).  A per-wave LDS tile
 // turns them into the accumulator layout and phase 1's V^T fragments, and
 // turns the result back for the store.  All loads are issued up front.
-template <int RT>
-__global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
-                                                         const Item* __restrict__ items) {
+template <int RT, int NW>
+__global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
+                                                             const Item* __restrict__ items) {
+  constexpr int NT = 64 * NW;
   const Item it = items[blockIdx.x];
   const ReflTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int m = t.m, nbj = t.nbj;
 
-  __shared__ double Wp[4][32][17];
+  __shared__ double Wp[NW][32][17];
   __shared__ double Wf[32][17];
   __shared__ double Tl[32][33];
-  __shared__ double Xt[4][16][33];  // per-wave transpose tile
+  __shared__ double Xt[NW][16][33];  // per-wave transpose tile
 
   // the slab column of register r in the row-major-lanes layout
   double* cp[4];
@@ -502,18 +505,19 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
     else if (vcol - t.w1 < t.w2)
       cp[r] = t.C2 + (size_t)(vcol - t.w1) * t.ldc;
   }
-  for (int e = tid; e < 32 * 32; e += 256) {
+  for (int e = tid; e < 32 * 32; e += NT) {
     const int i = e & 31, j = e >> 5;
     Tl[i][j] = (i < nbj && j < nbj) ? t.T[(size_t)j * t.ldt + i] : 0.0;
   }
-  // Registers hold V up front for m <= 512; beyond that V tiles are
-  // loaded when used (same layout).
-  constexpr bool PREV = RT <= 8;
+  // Registers hold V up front for up to 4 row tiles per wave; beyond that
+  // (8 waves x 8 tiles, m <= 1024) V tiles are loaded when used (same
+  // layout).
+  constexpr bool PREV = RT <= 4;
   double cl[RT][4];              // C[rt*16 + li][4r + lk]
   double vb[PREV ? RT : 1][8];   // V[rt*16 + li][4kq + lk]
 #pragma unroll
   for (int s = 0; s < RT; ++s) {
-    const int row = (wave + 4 * s) * 16 + li;
+    const int row = (wave + NW * s) * 16 + li;
 #pragma unroll
     for (int r = 0; r < 4; ++r) cl[s][r] = (cp[r] && row < m) ? cp[r][row] : 0.0;
     if constexpr (PREV) {
@@ -523,7 +527,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   }
   auto vtile = [&](int s, int kq) -> double {
     if constexpr (PREV) return vb[s][kq];
-    else return vload(t.V, t.ldv, m, nbj, (wave + 4 * s) * 16 + li, 4 * kq + lk);
+    else return vload(t.V, t.ldv, m, nbj, (wave + NW * s) * 16 + li, 4 * kq + lk);
   };
   double (*X)[33] = Xt[wave];
 
@@ -547,7 +551,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
       va[r][0] = X[4 * r + lk][li];
       va[r][1] = X[4 * r + lk][16 + li];
     }
-    if ((wave + 4 * s) * 16 < m) {
+    if ((wave + NW * s) * 16 < m) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (s & 1) {
@@ -569,9 +573,12 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   __syncthreads();
   // phase 2: Wf = -op(T) W
   __shared__ double W[32][17];
-  for (int e = tid; e < 32 * 16; e += 256) {
+  for (int e = tid; e < 32 * 16; e += NT) {
     const int i = e >> 4, j = e & 15;
-    W[i][j] = (Wp[0][i][j] + Wp[1][i][j]) + (Wp[2][i][j] + Wp[3][i][j]);
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v += Wp[w][i][j];
+    W[i][j] = v;
   }
   __syncthreads();
   // op(T) W on MFMA: waves 0 and 1 form rows 0-15 and 16-31 (K = 32).
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(256) void refl_apply_kernel(const ReflTask* __restr
   // phase 3: C += V Wf, then back to the row-major-lanes layout and out
 #pragma unroll
   for (int s = 0; s < RT; ++s) {
-    const int rt = wave + 4 * s;
+    const int rt = wave + NW * s;
     if (rt * 16 < m) {
 #pragma unroll
       for (int kq = 0; kq < 8; ++kq)
