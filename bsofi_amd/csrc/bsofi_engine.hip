@@ -105,7 +105,10 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   g.S = (2 * n + g.nb - 1) / g.nb;
   const int m = 2 * n;
   const int rpg = m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
-  g.panel_variant = rpg * 100 + g.nb / 4;  // panel_factor_kernel<RPG, CPW = nb / 4>
+  // panel_factor_kernel<RPG, CPW, NW>: 8 waves of nb / 8 columns for the
+  // 32-wide sub-panel of 2n in (256, 512], else 4 waves of nb / 4 columns
+  const int nw = (g.nb == 32 && m > 256 && m <= 512) ? 8 : 4;
+  g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
   g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti : (n <= 256 ? 2 : 1);
   g.scan_ntw = scan_ntw_for(n);
   return g;
@@ -642,12 +645,15 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       auto t = (const PanelTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
       // variant = RPG * 100 + CPW
-#define PANEL_CASE(R, C) \
-  case R * 100 + C: hipLaunchKernelGGL((panel_factor_kernel<R, C>), dim3(L.grid), dim3(256), 0, s, t, it); break;
+#define PANEL_CASE(R, C, W)                                                                       \
+  case W * 10000 + R * 100 + C:                                                                     \
+    hipLaunchKernelGGL((panel_factor_kernel<R, C, W>), dim3(L.grid), dim3(64 * W), 0, s, t, it); \
+    break;
       switch (L.variant) {
-        PANEL_CASE(2, 8) PANEL_CASE(4, 8) PANEL_CASE(8, 8)
-        PANEL_CASE(2, 4) PANEL_CASE(4, 4) PANEL_CASE(8, 4) PANEL_CASE(16, 4)
-        PANEL_CASE(2, 2) PANEL_CASE(4, 2) PANEL_CASE(8, 2) PANEL_CASE(16, 2) PANEL_CASE(32, 2)
+        PANEL_CASE(2, 8, 4) PANEL_CASE(4, 8, 4) PANEL_CASE(8, 8, 4)
+        PANEL_CASE(2, 4, 4) PANEL_CASE(4, 4, 4) PANEL_CASE(8, 4, 4) PANEL_CASE(16, 4, 4)
+        PANEL_CASE(2, 2, 4) PANEL_CASE(4, 2, 4) PANEL_CASE(8, 2, 4) PANEL_CASE(16, 2, 4) PANEL_CASE(32, 2, 4)
+        PANEL_CASE(8, 4, 8)
         default: break;
       }
 #undef PANEL_CASE

@@ -49,9 +49,9 @@ __device__ __forceinline__ d4 d4zero() {
 // ---------------------------------------------------------------------------
 // Sub-panel Householder QR (K1)
 // ---------------------------------------------------------------------------
-// 256 threads = 4 waves; wave w owns columns w CPW .. w CPW + CPW - 1 of
-// the sub-panel (NB = 4 CPW columns), lane l holds rows l + 64 q (q < RPG)
-// of them in registers.  Per column c:
+// NW waves (4 or 8); wave w owns columns w CPW .. w CPW + CPW - 1 of the
+// sub-panel (NB = NW CPW columns), lane l holds rows l + 64 q (q < RPG) of
+// them in registers.  Per column c:
 //   * the wave owning c has already updated it and put it in LDS (xs);
 //     one barrier, then every wave reads it (x) at its rows;
 //   * each wave forms, for its own columns j, sum_{r > c} x_r a_rj and
@@ -179,13 +179,13 @@ __device__ __forceinline__ double wave_max(double v) {
 // rounding.  Tl and X are LDS scratch of NB x (NB + 1).  Must be called by
 // every thread of the block.  Writes the nbj x nbj factor, Tout[c * ldt + i]
 // for i, c < nbj (ldt >= nbj may be below NB).
-template <int NB>
+template <int NB, int NT = 256>
 __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj,
                                         double* __restrict__ Tout, int ldt, double (*Tl)[NB + 1],
                                         double (*X)[NB + 1]) {
-  static_assert(NB * NB / 4 <= 256, "one thread per entry of the widest level");
+  static_assert(NB * NB / 4 <= NT, "one thread per entry of the widest level");
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
+  for (int e = tid; e < NB * NB; e += NT) {
     const int i = e / NB, j = e % NB;
     Tl[i][j] = (i == j) ? taus[i] : 0.0;
   }
@@ -212,7 +212,7 @@ __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double*
     }
     __syncthreads();
   }
-  for (int e = tid; e < nbj * nbj; e += 256) {
+  for (int e = tid; e < nbj * nbj; e += NT) {
     const int c = e / nbj, i = e % nbj;
     Tout[(size_t)c * ldt + i] = Tl[i][c];
   }
@@ -297,10 +297,11 @@ __device__ __forceinline__ Larfg dlarfg_scalar(double alpha, double xnorm) {
   return r;
 }
 
-template <int RPG, int CPW>
-__global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
-                                                           const Item* __restrict__ items) {
-  constexpr int NB = 4 * CPW;
+template <int RPG, int CPW, int NW>
+__global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
+                                                               const Item* __restrict__ items) {
+  constexpr int NB = NW * CPW;
+  constexpr int NT = 64 * NW;
   const Item it = items[blockIdx.x];
   const PanelTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
   auto run = [&](auto safe_tag) -> bool {
     constexpr bool SAFE = decltype(safe_tag)::value;
     bool bad = false;
-  for (int e = tid; e < NB * (NB + 1); e += 256) (&G[0][0])[e] = 0.0;
+  for (int e = tid; e < NB * (NB + 1); e += NT) (&G[0][0])[e] = 0.0;
   if (tid < NB) taus[tid] = 0.0;
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
     for (int q = 0; q < RPG; ++q) xs[0][q][lane] = a[q][0];
   }
 
-  for (int cw = 0; cw < 4; ++cw) {
+  for (int cw = 0; cw < NW; ++cw) {
 #pragma unroll
     for (int jc = 0; jc < CPW; ++jc) {
       const int c = cw * CPW + jc;
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(const PanelTask* __re
   }
   __syncthreads();
   if (tid < nbj) t.tau[tid] = taus[tid];
-  build_T<NB>(G, taus, nbj, t.T, t.ldt, Tl, Xl);
+  build_T<NB, NT>(G, taus, nbj, t.T, t.ldt, Tl, Xl);
 }
 
 // ---------------------------------------------------------------------------
