@@ -594,7 +594,26 @@ struct Planner {
         }
       }
       std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.cost > b.cost; });
-      for (auto& w : order) si.push_back({w.task, w.sub});
+      // XCD-aware order.  Workgroup i runs on XCD i mod 8, each XCD with its
+      // own L2.  Row panels of one block row (one cost) step through k in
+      // lockstep and read the same Q_k, so whole block rows are dealt to
+      // XCDs round-robin (balanced work, shared L2 lines), heaviest first.
+      constexpr int kXcd = 8;
+      std::vector<std::vector<Item>> q(kXcd);
+      int g = -1;
+      for (size_t i = 0; i < order.size(); ++i) {
+        if (i == 0 || order[i].cost != order[i - 1].cost || order[i].task != order[i - 1].task) ++g;
+        q[g % kXcd].push_back({order[i].task, order[i].sub});
+      }
+      std::vector<size_t> head(kXcd, 0);
+      for (size_t i = 0; i < order.size();) {
+        for (int x = 0; x < kXcd && i < order.size(); ++x) {
+          if (head[x] < q[x].size()) {
+            si.push_back(q[x][head[x]++]);
+            ++i;
+          }
+        }
+      }
       const int MT = 16 * key.first;
       int ldcar = MT + 1;
       if ((size_t)maxn * ldcar * sizeof(double) > (size_t)max_shmem) ldcar = MT;
