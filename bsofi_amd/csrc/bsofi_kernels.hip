@@ -46,6 +46,18 @@ __device__ __forceinline__ d4 d4zero() {
   return z;
 }
 
+// Global-memory views of the task pointers.  The pointers reach the kernels
+// inside task records, so the compiler cannot tell that they address global
+// memory and would emit flat loads and stores.  Flat operations count on the
+// LDS wait counter too: every wait for an LDS read would also wait for all
+// HBM loads in flight, and stores get a full drain in front of them.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> glb(T* p) {
+  return (gptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // Sub-panel Householder QR (K1)
 // ---------------------------------------------------------------------------
@@ -181,7 +193,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // for i, c < nbj (ldt >= nbj may be below NB).
 template <int NB, int NT = 256>
 __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj,
-                                        double* __restrict__ Tout, int ldt, double (*Tl)[NB + 1],
+                                        gptr<double> Tout, int ldt, double (*Tl)[NB + 1],
                                         double (*X)[NB + 1]) {
   static_assert(NB * NB / 4 <= NT, "one thread per entry of the widest level");
   const int tid = threadIdx.x;
@@ -307,6 +319,7 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = t.m, nbj = t.nbj, lda = t.lda;
   const int col0 = wave * CPW;
+  const gptr<double> Pg = glb(t.P);
 
   __shared__ double xs[2][RPG][64];  // the next column, double-buffered
   __shared__ double G[NB][NB + 1];
@@ -331,9 +344,14 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
       const int col = col0 + jj;
-      a[q][jj] = (row < m && col < nbj) ? t.P[(size_t)col * lda + row] : 0.0;
+      a[q][jj] = (row < m && col < nbj) ? Pg[(size_t)col * lda + row] : 0.0;
     }
   }
+  // vmcnt(0): the panel is in registers before anything else.  Without it the
+  // compiler cannot rule out loads still in flight at the final stores (a
+  // path with no column step) and drains the memory counter before each of
+  // them, which serialises the stores.
+  __builtin_amdgcn_s_waitcnt(0x0F70);
   if (wave == 0) {
 #pragma unroll
     for (int q = 0; q < RPG; ++q) xs[0][q][lane] = a[q][0];
@@ -446,12 +464,12 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
       const int col = col0 + jj;
-      if (row < m && col < nbj) t.P[(size_t)col * lda + row] = a[q][jj];
+      if (row < m && col < nbj) Pg[(size_t)col * lda + row] = a[q][jj];
     }
   }
   __syncthreads();
-  if (tid < nbj) t.tau[tid] = taus[tid];
-  build_T<NB, NT>(G, taus, nbj, t.T, t.ldt, Tl, Xl);
+  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
+  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Xl);
 }
 
 // ---------------------------------------------------------------------------
@@ -461,7 +479,7 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
 //   phase 1: W = V^T C        (nbj x 16, K = m split over the 4 waves)
 //   phase 2: W <- -op(T) W    (LDS, VALU)
 //   phase 3: C <- C + V W     (m x 16, K = nbj)
-__device__ __forceinline__ double vload(const double* __restrict__ V, int ldv, int m, int nbj, int row, int col) {
+__device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, int nbj, int row, int col) {
   if (row >= m || col >= nbj || row < col) return 0.0;
   if (row == col) return 1.0;
   return V[(size_t)col * ldv + row];
@@ -496,19 +514,20 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   __shared__ double Xt[NW][16][33];  // per-wave transpose tile
 
   // the slab column of register r in the row-major-lanes layout
-  double* cp[4];
+  gptr<double> cp[4];
+  const gptr<const double> Vg = glb(t.V);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int vcol = it.sub * 16 + 4 * r + lk;
     cp[r] = nullptr;
     if (vcol < t.w1)
-      cp[r] = t.C1 + (size_t)vcol * t.ldc;
+      cp[r] = glb(t.C1) + (size_t)vcol * t.ldc;
     else if (vcol - t.w1 < t.w2)
-      cp[r] = t.C2 + (size_t)(vcol - t.w1) * t.ldc;
+      cp[r] = glb(t.C2) + (size_t)(vcol - t.w1) * t.ldc;
   }
   for (int e = tid; e < 32 * 32; e += NT) {
     const int i = e & 31, j = e >> 5;
-    Tl[i][j] = (i < nbj && j < nbj) ? t.T[(size_t)j * t.ldt + i] : 0.0;
+    Tl[i][j] = (i < nbj && j < nbj) ? glb(t.T)[(size_t)j * t.ldt + i] : 0.0;
   }
   // Registers hold V up front for up to 4 row tiles per wave; beyond that
   // (8 waves x 8 tiles, m <= 1024) V tiles are loaded when used (same
@@ -523,12 +542,12 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     for (int r = 0; r < 4; ++r) cl[s][r] = (cp[r] && row < m) ? cp[r][row] : 0.0;
     if constexpr (PREV) {
 #pragma unroll
-      for (int kq = 0; kq < 8; ++kq) vb[s][kq] = vload(t.V, t.ldv, m, nbj, row, 4 * kq + lk);
+      for (int kq = 0; kq < 8; ++kq) vb[s][kq] = vload(Vg, t.ldv, m, nbj, row, 4 * kq + lk);
     }
   }
   auto vtile = [&](int s, int kq) -> double {
     if constexpr (PREV) return vb[s][kq];
-    else return vload(t.V, t.ldv, m, nbj, (wave + NW * s) * 16 + li, 4 * kq + lk);
+    else return vload(Vg, t.ldv, m, nbj, (wave + NW * s) * 16 + li, 4 * kq + lk);
   };
   double (*X)[33] = Xt[wave];
 
@@ -643,7 +662,7 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
     const int row = k0 + lk;
     double f[2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) f[a] = (row < ke) ? vload(t.P, t.lda, m, nbj, row, a * 16 + li) : 0.0;
+    for (int a = 0; a < 2; ++a) f[a] = (row < ke) ? vload(glb((const double*)t.P), t.lda, m, nbj, row, a * 16 + li) : 0.0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -655,14 +674,14 @@ __global__ __launch_bounds__(256) void larft_kernel(const PanelTask* __restrict_
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) Gp[wave][a * 16 + crow(lk, r)][b * 16 + li] = acc[a][b][r];
-  if (tid < 32) taus[tid] = (tid < nbj) ? t.tau[tid] : 0.0;
+  if (tid < 32) taus[tid] = (tid < nbj) ? glb(t.tau)[tid] : 0.0;
   __syncthreads();
   for (int e = tid; e < 32 * 32; e += 256) {
     const int i = e >> 5, j = e & 31;
     G[i][j] = (Gp[0][i][j] + Gp[1][i][j]) + (Gp[2][i][j] + Gp[3][i][j]);
   }
   __syncthreads();
-  build_T<32>(G, taus, nbj, t.T, t.ldt, &Gp[0][0], &Gp[1][0]);
+  build_T<32>(G, taus, nbj, glb(t.T), t.ldt, &Gp[0][0], &Gp[1][0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -683,9 +702,9 @@ __global__ __launch_bounds__(256) void det_kernel(const DetTask* __restrict__ ta
   double lg = 0.0;
   int flips = 0, zero = 0;
   for (int i = tid; i < t.N; i += 256) {
-    const double r = t.A[(size_t)i * t.lda + i];
+    const double r = glb(t.A)[(size_t)i * t.lda + i];
     zero |= (r == 0.0);
-    flips ^= (r < 0.0) ^ (t.tau[i] != 0.0);
+    flips ^= (r < 0.0) ^ (glb(t.tau)[i] != 0.0);
     lg += log(fabs(r));
   }
   ls[tid] = lg;
@@ -699,8 +718,8 @@ __global__ __launch_bounds__(256) void det_kernel(const DetTask* __restrict__ ta
     __syncthreads();
   }
   if (tid == 0) {
-    t.out[0] = (par[0] & 2) ? 0.0 : ((par[0] & 1) ? -1.0 : 1.0);
-    t.out[1] = ls[0];
+    glb(t.out)[0] = (par[0] & 2) ? 0.0 : ((par[0] & 1) ? -1.0 : 1.0);
+    glb(t.out)[1] = ls[0];
   }
 }
 
@@ -732,6 +751,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
   const int i0 = tm * TM, j0 = tn * TM;
   const int wi = wave & 1, wj = wave >> 1;
   const int M = t.M, N = t.N, K = t.K;
+  const gptr<const double> Ag = glb(t.A), Bg = glb(t.B), Dg = glb(t.D);
+  const gptr<double> Cg = glb(t.C);
 
   __shared__ double As[2][TM][17];
   __shared__ double Bs[2][TM][17];
@@ -743,12 +764,12 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
       const int gi = i0 + ia, gk = k0 + ka + AK * r;
-      ra[r] = (gi < M && gk < K && !(t.amask && gi > gk)) ? t.A[(size_t)gk * t.lda + gi] : 0.0;
+      ra[r] = (gi < M && gk < K && !(t.amask && gi > gk)) ? Ag[(size_t)gk * t.lda + gi] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < NBV; ++r) {
       const int gk = k0 + kb, gj = j0 + jb + 16 * r;
-      rb[r] = (gk < K && gj < N && !(t.bmask && gk > gj)) ? t.B[(size_t)gj * t.ldb + gk] : 0.0;
+      rb[r] = (gk < K && gj < N && !(t.bmask && gk > gj)) ? Bg[(size_t)gj * t.ldb + gk] : 0.0;
     }
   };
   auto stash = [&](int buf) {
@@ -797,9 +818,9 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
         const int row = i0 + wi * 16 * WT + a * 16 + crow(lk, r);
         if (row >= M) continue;
         double v = t.alpha * acc[a][b][r];
-        double* cp = t.C + (size_t)col * t.ldc + row;
+        const gptr<double> cp = Cg + (size_t)col * t.ldc + row;
         if (t.beta != 0.0) v = fma(t.beta, *cp, v);
-        if (t.D && col >= t.dcol0) v += t.D[(size_t)(col - t.dcol0) * t.ldd + row];
+        if (t.D && col >= t.dcol0) v += Dg[(size_t)(col - t.dcol0) * t.ldd + row];
         *cp = v;
       }
     }
@@ -818,7 +839,7 @@ __global__ __launch_bounds__(64) void trtri_diag_kernel(const TrtriTask* __restr
   if (tid == 0) bad = 0;
   for (int e = tid; e < 32 * 32; e += 64) {
     const int i = e & 31, j = e >> 5;
-    S[i][j] = (i <= j && i < nb && j < nb) ? t.X[(size_t)j * t.ld + i] : 0.0;
+    S[i][j] = (i <= j && i < nb && j < nb) ? glb(t.X)[(size_t)j * t.ld + i] : 0.0;
   }
   __syncthreads();
   if (tid < nb && S[tid][tid] == 0.0) {
@@ -842,7 +863,7 @@ __global__ __launch_bounds__(64) void trtri_diag_kernel(const TrtriTask* __restr
   }
   for (int e = tid; e < 32 * 32; e += 64) {
     const int i = e & 31, j = e >> 5;
-    if (i <= j && j < nb) t.X[(size_t)j * t.ld + i] = S[i][j];
+    if (i <= j && j < nb) glb(t.X)[(size_t)j * t.ld + i] = S[i][j];
   }
 }
 
@@ -859,7 +880,7 @@ __global__ void eye_kernel(const EyeTask* __restrict__ tasks, int ntasks) {
          e += (long long)gridDim.x * blockDim.x) {
       const long long mi = e / per, r = e - mi * per;
       const int j = (int)(r / t.dim), i = (int)(r - (long long)j * t.dim);
-      t.Q[mi * t.stride + (long long)j * t.dim + i] = (i == j) ? 1.0 : 0.0;
+      glb(t.Q)[mi * t.stride + (long long)j * t.dim + i] = (i == j) ? 1.0 : 0.0;
     }
   }
 }
@@ -889,7 +910,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   const int li = lane & 15, lk = lane >> 4;
   const int n = t.n, p = t.p, lda = t.lda, N = n * p, n2 = 2 * n;
   const int R0 = t.r0 + it.sub * MT;
-  double* __restrict__ A = t.A;
+  const gptr<double> A = glb(t.A);
 
   // rows served by this lane in A fragments: R0 + a*16 + li
   int rrow[MTI], rblk[MTI], rin[MTI];
@@ -920,7 +941,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
 #pragma unroll
   for (int b = 0; b < NTW; ++b) bcol[b] = (wave + 4 * b) * 16 + li;
   for (int k = p - 2; k >= kstop; --k) {
-    const double* __restrict__ Qk = t.Q + (size_t)k * t.qstride;
+    const gptr<const double> Qk = glb(t.Q) + (size_t)k * t.qstride;
     d4 acc[MTI][NTW];
 #pragma unroll
     for (int a = 0; a < MTI; ++a)
@@ -932,7 +953,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
     double a0[MTI], b0[NTW], a1[MTI], b1[NTW];
     if (rbmin <= k) {
       // [X_{r,k}] part: K rows 0..n-1 of Q_k^T
-      const double* __restrict__ Xk = A + (size_t)k * n * lda;
+      const gptr<const double> Xk = A + (size_t)k * n * lda;
       auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
         const int kk = k0 + lk;
 #pragma unroll
@@ -962,7 +983,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
     }
     // [carry] part: K rows n..2n-1 of Q_k^T
     {
-      const double* __restrict__ Qc = Qk + (size_t)n * n2;
+      const gptr<const double> Qc = Qk + (size_t)n * n2;
       auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
         const int kk = k0 + lk;
 #pragma unroll
