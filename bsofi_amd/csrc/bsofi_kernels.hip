@@ -64,16 +64,17 @@ __device__ __forceinline__ gptr<T> glb(T* p) {
 // NW waves (4 or 8); wave w owns columns w CPW .. w CPW + CPW - 1 of the
 // sub-panel (NB = NW CPW columns), lane l holds rows l + 64 q (q < RPG) of
 // them in registers.  Per column c:
-//   * the wave owning c has already updated it and put it in LDS (xs);
-//     one barrier, then every wave reads it (x) at its rows;
-//   * each wave forms, for its own columns j, sum_{r > c} x_r a_rj and
-//     ||x||^2 and reduces them over its 64 lanes (permlane swaps + DPP):
-//       j == c : ||x||^2 (dlarfg),  j > c : H_c's dot products,
-//       j < c  : v_j^T v_c (the T factor, dlarft);
-//   * every wave computes the same dlarfg from the same values, applies
-//     H_c to its columns, and the owner of c + 1 updates that column first
-//     and publishes it for the next step.
-// So a column costs one barrier and no cross-wave reduction.
+//   * the wave owning c has formed its reflector (dlarfg) and put v_c
+//     (zero above row c, one at row c) and tau_c in LDS; one barrier, then
+//     every wave reads v_c at its rows;
+//   * each wave forms d_j = v_c^T a_j for its own columns j and reduces them
+//     over its 64 lanes (permlane swaps + DPP):  j > c : H_c's dot products,
+//     j < c : v_j^T v_c (the Gram entries of the T factor, dlarft);
+//   * the owner of c + 1 applies H_c to that column first, forms its
+//     reflector (norm reduction + dlarfg) and publishes it, then every wave
+//     applies H_c to the rest of its unfinished columns.
+// So a column costs one barrier, no cross-wave reduction, and only the
+// owner of the next column spends instructions on its norm and dlarfg.
 
 // 64-bit DPP move (two 32-bit halves); CTRL is a GFX9 DPP control code.
 template <int CTRL>
@@ -309,6 +310,16 @@ __device__ __forceinline__ Larfg dlarfg_scalar(double alpha, double xnorm) {
   return r;
 }
 
+// Entry of the CPW transposed-reduced values that lane l holds after
+// wave_reduce_stage<CPW, 0>: its bits 32, 16, ... of the halving stages.
+template <int CPW>
+__device__ __forceinline__ int wave_reduce_entry(int lane) {
+  int jj = 0, bitv = 32;
+  for (int h = CPW / 2; h >= 1; h /= 2, bitv /= 2)
+    if (lane & bitv) jj |= h;
+  return jj;
+}
+
 template <int RPG, int CPW, int NW>
 __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
                                                                const Item* __restrict__ items) {
@@ -316,12 +327,13 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
   constexpr int NT = 64 * NW;
   const Item it = items[blockIdx.x];
   const PanelTask t = tasks[it.task];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int m = t.m, nbj = t.nbj, lda = t.lda;
   const int col0 = wave * CPW;
   const gptr<double> Pg = glb(t.P);
 
-  __shared__ double xs[2][RPG][64];  // the next column, double-buffered
+  __shared__ double vs[2][RPG][64];  // v of the current column, double-buffered
   __shared__ double G[NB][NB + 1];
   __shared__ double taus[NB];
   __shared__ double Tl[NB][NB + 1];
@@ -329,135 +341,138 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
 
   double a[RPG][CPW];
   // The column loop runs first on dlarfg's fast path alone, with no branch
-  // in it (a branch on the norm would stall every column on the norm's
-  // reduction); `bad` records whether any column left the fast path's range.
+  // in it; `bad` records whether any column left the fast path's range.
   // Then, and only then, the panel -- still intact in memory, it is written
   // back at the end -- is factored again with LAPACK's scaled path.
   auto run = [&](auto safe_tag) -> bool {
     constexpr bool SAFE = decltype(safe_tag)::value;
     bool bad = false;
-  for (int e = tid; e < NB * (NB + 1); e += NT) (&G[0][0])[e] = 0.0;
-  if (tid < NB) taus[tid] = 0.0;
+    for (int e = tid; e < NB * (NB + 1); e += NT) (&G[0][0])[e] = 0.0;
+    if (tid < NB) taus[tid] = 0.0;
 #pragma unroll
-  for (int q = 0; q < RPG; ++q) {
-    const int row = lane + 64 * q;
+    for (int q = 0; q < RPG; ++q) {
+      const int row = lane + 64 * q;
 #pragma unroll
-    for (int jj = 0; jj < CPW; ++jj) {
-      const int col = col0 + jj;
-      a[q][jj] = (row < m && col < nbj) ? Pg[(size_t)col * lda + row] : 0.0;
-    }
-  }
-  // vmcnt(0): the panel is in registers before anything else.  Without it the
-  // compiler cannot rule out loads still in flight at the final stores (a
-  // path with no column step) and drains the memory counter before each of
-  // them, which serialises the stores.
-  __builtin_amdgcn_s_waitcnt(0x0F70);
-  if (wave == 0) {
-#pragma unroll
-    for (int q = 0; q < RPG; ++q) xs[0][q][lane] = a[q][0];
-  }
-
-  for (int cw = 0; cw < NW; ++cw) {
-#pragma unroll
-    for (int jc = 0; jc < CPW; ++jc) {
-      const int c = cw * CPW + jc;
-      if (c < nbj) {
-        const int buf = c & 1;
-        __syncthreads();
-        double x[RPG];
-#pragma unroll
-        for (int q = 0; q < RPG; ++q) x[q] = xs[buf][q][lane];
-        const double alpha = readlane64(x[0], c);  // row c < 64 sits in lane c, q = 0
-        x[0] = (lane > c) ? x[0] : 0.0;            // zero at and above the pivot
-        double pr[CPW];                             // pivot row of my columns
-#pragma unroll
-        for (int jj = 0; jj < CPW; ++jj) pr[jj] = readlane64(a[0][jj], c);
-
-        // ||x||^2 and dlarfg come first and without a branch, so their
-        // dependent chain can interleave with the dot products below.
-        double nrm = x[0] * x[0];
-#pragma unroll
-        for (int q = 1; q < RPG; ++q) nrm = fma(x[q], x[q], nrm);
-        const double xn2 = wave_sum(nrm, lane);
-        // dlarfg.  In range, beta = -sign(alpha) sqrt(alpha^2 + ||x||^2)
-        // and, with r = 1 / (beta (alpha - beta)),
-        //   tau = (beta - alpha) / beta = -(alpha - beta)^2 r,  scale = beta r.
-        // A sum of squares outside [2^-960, 2^960] (including x = 0) or
-        // |alpha| >= 2^480 takes LAPACK's scaled path instead.
-        const bool inrange = xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480;
-        double beta = -copysign(sqrt_normal(fma(alpha, alpha, xn2)), alpha);
-        const double amb = alpha - beta;
-        const double rr = rcp_normal(beta * amb);
-        double tau = -(amb * amb) * rr;
-        double scale = beta * rr;
-
-        double P[CPW];
-#pragma unroll
-        for (int jj = 0; jj < CPW; ++jj) {
-          double acc = x[0] * a[0][jj];
-#pragma unroll
-          for (int q = 1; q < RPG; ++q) acc = fma(x[q], a[q][jj], acc);
-          P[jj] = acc;
-        }
-        wave_reduce_stage<CPW, 0, CPW>(P, lane);
-        double tot[CPW];
-#pragma unroll
-        for (int jj = 0; jj < CPW; ++jj) tot[jj] = readlane64(P[0], wave_reduce_lane<CPW>(jj));
-        if constexpr (SAFE) {
-          if (!inrange) {
-            const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
-            tau = lf.tau;
-            beta = lf.beta;
-            scale = lf.scale;
-          }
-        } else {
-          bad |= !inrange;
-        }
-
-        // w_j = tau v_c^T a_j (j > c); G[j][c] = v_j^T v_c (j < c)
-        double w[CPW];
-#pragma unroll
-        for (int jj = 0; jj < CPW; ++jj) {
-          const int j = col0 + jj;
-          const double d = fma(scale, tot[jj], pr[jj]);
-          w[jj] = (j > c) ? tau * d : 0.0;
-          if (lane == jj) G[j][c] = (j < c) ? d : 0.0;
-        }
-        double v[RPG];
-        v[0] = (lane > c) ? x[0] * scale : (lane == c ? 1.0 : 0.0);
-#pragma unroll
-        for (int q = 1; q < RPG; ++q) v[q] = x[q] * scale;
-        // slot of column c + 1 first; its owner publishes it
-        const int jn = (jc + 1) % CPW;
-        const int wn = cw + (jc + 1) / CPW;
-#pragma unroll
-        for (int q = 0; q < RPG; ++q) a[q][jn] = fma(-v[q], w[jn], a[q][jn]);
-        if (wave == wn) {
-#pragma unroll
-          for (int q = 0; q < RPG; ++q) xs[buf ^ 1][q][lane] = a[q][jn];
-        }
-#pragma unroll
-        for (int jj = 0; jj < CPW; ++jj) {
-          if (jj == jn) continue;
-#pragma unroll
-          for (int q = 0; q < RPG; ++q) a[q][jj] = fma(-v[q], w[jj], a[q][jj]);
-        }
-        if (wave == cw) {  // column c becomes the reflector (beta on the diagonal)
-          a[0][jc] = (lane > c) ? v[0] : (lane == c ? beta : a[0][jc]);
-#pragma unroll
-          for (int q = 1; q < RPG; ++q) a[q][jc] = v[q];
-        }
-        if (tid == 0) taus[c] = tau;
+      for (int jj = 0; jj < CPW; ++jj) {
+        const int col = col0 + jj;
+        a[q][jj] = (row < m && col < nbj) ? Pg[(size_t)col * lda + row] : 0.0;
       }
     }
-  }
-    return bad;
-  };
-  if (run(std::false_type{})) {
-    __syncthreads();  // every wave is done with xs before it is refilled
-    run(std::true_type{});
-  }
+    // vmcnt(0): the panel is in registers before anything else.  Without it
+    // the compiler cannot rule out loads still in flight at the final
+    // stores (a path with no column step) and drains the memory counter
+    // before each of them, which serialises the stores.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 
+    // Reflector of column c, local column jc of the calling wave, whose
+    // entries at and below row c are final: dlarfg, v_c to vs[buf], tau_c
+    // to taus[c]; the column becomes beta on the diagonal and v below it.
+    auto reflector = [&](int jc, int c, int buf) {
+      const double alpha = readlane64(a[0][jc], c);  // row c < 64: lane c, q = 0
+      double x[RPG];
+      x[0] = (lane > c) ? a[0][jc] : 0.0;
+#pragma unroll
+      for (int q = 1; q < RPG; ++q) x[q] = a[q][jc];
+      double n0 = x[0] * x[0], n1 = 0.0;
+#pragma unroll
+      for (int q = 1; q < RPG; ++q) {
+        if (q & 1) n1 = fma(x[q], x[q], n1);
+        else n0 = fma(x[q], x[q], n0);
+      }
+      const double xn2 = wave_sum(n0 + n1, lane);
+      // In range, beta = -sign(alpha) sqrt(alpha^2 + ||x||^2) and, with
+      // r = 1 / (beta (alpha - beta)),
+      //   tau = (beta - alpha) / beta = -(alpha - beta)^2 r,  scale = beta r.
+      // A sum of squares outside [2^-960, 2^960] (including x = 0) or
+      // |alpha| >= 2^480 takes LAPACK's scaled path instead.
+      const bool inrange = xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480;
+      double beta = -copysign(sqrt_normal(fma(alpha, alpha, xn2)), alpha);
+      const double amb = alpha - beta;
+      const double rr = rcp_normal(beta * amb);
+      double tau = -(amb * amb) * rr;
+      double scale = beta * rr;
+      if constexpr (SAFE) {
+        if (!inrange) {
+          const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
+          tau = lf.tau;
+          beta = lf.beta;
+          scale = lf.scale;
+        }
+      } else {
+        bad |= !inrange;
+      }
+      const double v0 = (lane > c) ? x[0] * scale : (lane == c ? 1.0 : 0.0);
+      vs[buf][0][lane] = v0;
+      a[0][jc] = (lane > c) ? v0 : (lane == c ? beta : a[0][jc]);
+#pragma unroll
+      for (int q = 1; q < RPG; ++q) {
+        const double v = x[q] * scale;
+        vs[buf][q][lane] = v;
+        a[q][jc] = v;
+      }
+      if (lane == 0) taus[c] = tau;
+    };
+
+    if (wave == 0 && nbj > 0) reflector(0, 0, 0);
+    for (int cw = 0; cw < NW; ++cw) {
+#pragma unroll
+      for (int jc = 0; jc < CPW; ++jc) {
+        const int c = cw * CPW + jc;
+        if (c < nbj) {
+          const int buf = c & 1;
+          __syncthreads();
+          double v[RPG];
+#pragma unroll
+          for (int q = 0; q < RPG; ++q) v[q] = vs[buf][q][lane];
+          const double tau = taus[c];
+          // d_j = v_c^T a_j (two chains per column)
+          double P[CPW];
+#pragma unroll
+          for (int jj = 0; jj < CPW; ++jj) {
+            double s0 = v[0] * a[0][jj], s1 = 0.0;
+#pragma unroll
+            for (int q = 1; q < RPG; ++q) {
+              if (q & 1) s1 = fma(v[q], a[q][jj], s1);
+              else s0 = fma(v[q], a[q][jj], s0);
+            }
+            P[jj] = s0 + s1;
+          }
+          wave_reduce_stage<CPW, 0, CPW>(P, lane);
+          // Gram entries v_j^T v_c of this wave's finished columns j < c
+          {
+            const int jj = wave_reduce_entry<CPW>(lane);
+            if ((lane & (64 / CPW - 1)) == 0 && col0 + jj < c) G[col0 + jj][c] = P[0];
+          }
+          double w[CPW];
+#pragma unroll
+          for (int jj = 0; jj < CPW; ++jj) w[jj] = tau * readlane64(P[0], wave_reduce_lane<CPW>(jj));
+          // column c + 1 first; its owner forms and publishes its reflector
+          const int jn = (jc + 1) % CPW;
+          const int wn = cw + (jc + 1) / CPW;
+          if (col0 + jn > c && col0 + jn < nbj) {
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) a[q][jn] = fma(-v[q], w[jn], a[q][jn]);
+          }
+          if (wave == wn && c + 1 < nbj) reflector(jn, c + 1, buf ^ 1);
+#pragma unroll
+          for (int jj = 0; jj < CPW; ++jj) {
+            if (jj == jn) continue;
+            if (col0 + jj > c && col0 + jj < nbj) {
+#pragma unroll
+              for (int q = 0; q < RPG; ++q) a[q][jj] = fma(-v[q], w[jj], a[q][jj]);
+            }
+          }
+        }
+      }
+    }
+    return __syncthreads_or(bad) != 0;
+  };
+  if (run(std::false_type{})) run(std::true_type{});
+
+  // G and taus are complete (the final barrier of run); T first, so that the
+  // barriers inside build_T do not come after the stores.
+  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
+  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Xl);
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
     const int row = lane + 64 * q;
@@ -467,9 +482,6 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
       if (row < m && col < nbj) Pg[(size_t)col * lda + row] = a[q][jj];
     }
   }
-  __syncthreads();
-  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
-  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Xl);
 }
 
 // ---------------------------------------------------------------------------

@@ -84,6 +84,35 @@ def test_phases_match_oracle(B, O, kind, a, p, eps):
     assert np.array_equal(Gg2, Gg)
 
 
+@pytest.mark.parametrize("scale", [1e-300, 1e-160, 1e160])
+def test_scaled_inputs_match_oracle(B, O, scale):
+    """Entries whose squares leave the panel's fast dlarfg range (sums of
+    squares outside [2^-960, 2^960]) take LAPACK's scaled path; 1e-300 also
+    runs dlarfg's safe-minimum rescaling loop (|beta| < safmin)."""
+    A0, n = make("rand_eye", 17, 4)
+    p = 4
+    N = n * p
+    A0 = A0 * scale
+    print(f"case scaled {scale:g} n={n} p={p}", flush=True)
+
+    def srel(a, b):  # relative error, norms taken after rescaling by max|b|
+        s = np.abs(b).max()
+        return rel(a / s, b / s)
+
+    Ao = A0.copy(order="F")
+    tau_o = np.zeros(N)
+    assert O.bsof(n, p, Ao, tau_o) == 0
+    Ag = A0.copy(order="F")
+    tau_g = np.zeros(N)
+    assert B.lapackXbsof(n, p, Ag, tau_g) == 0
+    assert np.all(np.isfinite(Ag)) and np.all(np.isfinite(tau_g))
+    assert rel(tau_g, tau_o) <= 1e-10
+    # V (below the diagonal) is scale-free, R (on and above it) carries the scale
+    low = np.tril(np.ones((N, N), dtype=bool), -1)
+    assert srel(np.where(low, Ag, 0.0), np.where(low, Ao, 0.0)) <= 1e-10
+    assert srel(np.where(low, 0.0, Ag), np.where(low, 0.0, Ao)) <= 1e-10
+
+
 @pytest.mark.parametrize("path", GOLD_CASES, ids=[os.path.basename(p)[:-4] for p in GOLD_CASES])
 def test_golden_fixtures_through_handle_api(B, path, eps):
     g = np.load(path)
