@@ -88,12 +88,21 @@ static bool nb_supported(int n, int nb) {
 
 static int scan_ntw_for(int n) { return ((2 * n + 15) / 16 + 3) / 4; }
 
+// Scan forms: scan_mti = 1, 2, 4 is bsoi_scan_kernel (16x16x4 MFMA) on
+// 16 * scan_mti rows; kScan4 is bsoi_scan4_kernel (4x4x4 MFMA) on 32 rows.
+static constexpr int kScan4 = 9;
+static int scan_rows(int mti) { return mti == kScan4 ? 32 : 16 * mti; }
+
 static bool scan_variant_ok(int mti, int ntw) {
+  if (mti == kScan4) return ntw >= 1 && ntw <= 8;  // with 8 | n (scan_ok)
   if (mti == 1) return ntw >= 1 && ntw <= 16;
   if (mti == 2) return ntw >= 1 && ntw <= 8;
   if (mti == 4) return ntw >= 1 && ntw <= 4;
   return false;
 }
+
+// bsoi_scan4_kernel reads Q_k^T without column masks: 16 | 2n.
+static bool scan_ok(int n, int mti, int ntw) { return scan_variant_ok(mti, ntw) && (mti != kScan4 || n % 8 == 0); }
 
 static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   Geo g;
@@ -109,7 +118,7 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   // 32-wide sub-panel of 2n in (256, 512], else 4 waves of nb / 4 columns
   const int nw = (g.nb == 32 && m > 256 && m <= 512) ? 8 : 4;
   g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
-  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti : (n <= 256 ? 2 : 1);
+  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti : (n <= 256 ? (n % 8 == 0 ? kScan4 : 2) : 1);
   g.scan_ntw = scan_ntw_for(n);
   return g;
 }
@@ -582,7 +591,7 @@ struct Planner {
       for (auto& M : mats) {
         const Geo& g = M.g;
         if (g.scan_mti != key.first || g.scan_ntw != key.second) continue;
-        const int MT = 16 * g.scan_mti;
+        const int MT = scan_rows(g.scan_mti);
         const int id = (int)st.size();
         st.push_back({M.A, M.Q, (long long)4 * g.n * g.n, g.lda, g.n, g.p, 0, g.N, diag_only ? 1 : 0});
         maxn = std::max(maxn, g.n);
@@ -614,7 +623,7 @@ struct Planner {
           }
         }
       }
-      const int MT = 16 * key.first;
+      const int MT = scan_rows(key.first);
       int ldcar = MT + 1;
       if ((size_t)maxn * ldcar * sizeof(double) > (size_t)max_shmem) ldcar = MT;
       emit(F_SCAN, key.first * 100 + key.second, st, si, ldcar, (size_t)maxn * ldcar * sizeof(double));
@@ -638,11 +647,30 @@ static void set_scan_attr(size_t shmem) {
                               (int)shmem);
 }
 
-// scan instances: variant = MTI * 100 + NTW (see scan_variant_ok)
+template <int W>
+static void launch_scan4(const Launch& L, const char* base, hipStream_t s) {
+  hipLaunchKernelGGL((bsoi_scan4_kernel<W, 4>), dim3(L.grid), dim3(256), L.shmem, s,
+                     (const ScanTask*)(base + L.task_off), (const Item*)(base + L.item_off), L.ldcar);
+}
+
+template <int W>
+static void set_scan_attr4(size_t shmem) {
+  if (shmem > 65536)
+    (void)hipFuncSetAttribute((const void*)bsoi_scan4_kernel<W, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)shmem);
+}
+
+// scan instances: variant = MTI * 100 + NTW (see scan_variant_ok); the
+// 4x4x4 form is kScan4 * 100 + NTW
 #define SCAN_C(FN, M, W, ...) \
   case M * 100 + W: FN<M, W>(__VA_ARGS__); break;
+#define SCAN4_C(FN, W, ...) \
+  case kScan4 * 100 + W: FN##4<W>(__VA_ARGS__); break;
 #define SCAN_SWITCH(FN, ...)                                                                                 \
   switch (variant) {                                                                                         \
+    SCAN4_C(FN, 1, __VA_ARGS__) SCAN4_C(FN, 2, __VA_ARGS__) SCAN4_C(FN, 3, __VA_ARGS__)                      \
+    SCAN4_C(FN, 4, __VA_ARGS__) SCAN4_C(FN, 5, __VA_ARGS__) SCAN4_C(FN, 6, __VA_ARGS__)                      \
+    SCAN4_C(FN, 7, __VA_ARGS__) SCAN4_C(FN, 8, __VA_ARGS__)                                                  \
     SCAN_C(FN, 1, 1, __VA_ARGS__) SCAN_C(FN, 1, 2, __VA_ARGS__) SCAN_C(FN, 1, 3, __VA_ARGS__)                \
     SCAN_C(FN, 1, 4, __VA_ARGS__) SCAN_C(FN, 1, 5, __VA_ARGS__) SCAN_C(FN, 1, 6, __VA_ARGS__)                \
     SCAN_C(FN, 1, 7, __VA_ARGS__) SCAN_C(FN, 1, 8, __VA_ARGS__) SCAN_C(FN, 1, 9, __VA_ARGS__)                \
@@ -739,7 +767,7 @@ int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti) {
   t.nb = nb;
   t.scan_mti = scan_mti;
   const Geo g = geometry(n, 2, 2 * n, &t);
-  if (!nb_supported(n, g.nb) || !scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+  if (!nb_supported(n, g.nb) || !scan_ok(n, g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
   if (nb == 0 && scan_mti == 0)
     ctx->tune.erase(n);
   else
@@ -867,7 +895,7 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     if (a) return a;
     if (!mats[i].A || !mats[i].tau) return -3;
     const Geo g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
-    if (!nb_supported(g.n, g.nb) || !scan_variant_ok(g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+    if (!nb_supported(g.n, g.nb) || !scan_ok(g.n, g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
   }
   HIPCHK(hipSetDevice(ctx->device));
   auto* P = new bsofi_plan_s();

@@ -1055,6 +1055,191 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
 }
 
 // ---------------------------------------------------------------------------
+// BSOI row-panel scan on v_mfma_f64_4x4x4f64 (K5, 4x4x4 form).
+// ---------------------------------------------------------------------------
+// The same scan as bsoi_scan_kernel for a 32-row panel, on the 4-block
+// 4x4x4 FP64 MFMA (measured here at 1.5x the FP64 rate of the 16x16x4
+// form).  For lane l = 16 x + 4 blk + y the instruction takes
+//   A: A_blk[y][x],  B: B_blk[x][y],  and accumulates D_blk[x][y],
+// the four blocks blk independent (checked on the device by the layout
+// probe behind this kernel and by the parity tests).  Here all four blocks
+// share the same 4 rows and cover the four 4-column quarters of a 16-column
+// tile: B is then exactly the 16x16x4 B fragment (Q_k^T read as in
+// bsoi_scan_kernel), and A is the 4 x 4 piece of the rows (the same in the
+// four blocks).  NW waves; wave w holds 8 row groups x NTW column tiles
+// w, w + NW, ... of accumulators, one double each (64 at NW = 4, in AGPRs:
+// one workgroup per CU, one wave per SIMD, which measured faster than two
+// waves per SIMD with half the tiles each, as each A value then feeds 8
+// MFMAs).  The carry sits in LDS with row 4 g + y at y * 8 + g, so that a
+// lane's 8 A values of a k-step are contiguous.
+template <int NTW, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* __restrict__ tasks,
+                                                                const Item* __restrict__ items, int ldcar) {
+  constexpr int NT = 64 * NW;
+  constexpr int MT = 32;
+  extern __shared__ double carry[];  // carry[kk * ldcar + (i & 3) * 8 + (i >> 2)] = row i, column kk
+  const Item it = items[blockIdx.x];
+  const ScanTask t = tasks[it.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x = lane >> 4, y = lane & 3, l16 = lane & 15;
+  const int n = t.n, p = t.p, lda = t.lda, N = n * p, n2 = 2 * n;
+  const int R0 = t.r0 + it.sub * MT;
+  const gptr<double> A = glb(t.A);
+  auto cidx = [&](int kk, int i) { return kk * ldcar + (i & 3) * 8 + (i >> 2); };
+
+  // rows of this lane's A values: R0 + 4 g + y
+  int rblk[8], rin[8];
+#pragma unroll
+  for (int g = 0; g < 8; ++g) {
+    const int row = R0 + 4 * g + y;
+    rblk[g] = row < N ? row / n : p;
+    rin[g] = row - rblk[g] * n;
+  }
+  const int ntiles = (n2 + 15) / 16;
+
+  for (int e = tid; e < n * MT; e += NT) {
+    const int i = e % MT, kk = e / MT;
+    const int row = R0 + i;
+    double v = 0.0;
+    if (row < N) {
+      const int rb = row / n;
+      if (!(rb == p - 1 && row - rb * n > kk)) v = A[(size_t)((p - 1) * n + kk) * lda + row];
+    }
+    carry[cidx(kk, i)] = v;
+  }
+  __syncthreads();
+
+  const int rbmin = R0 / n;
+  const bool diag = t.diag_only != 0;
+  const int kstop = diag ? max(rbmin - 1, 0) : 0;
+  int bcol[NTW];
+#pragma unroll
+  for (int b = 0; b < NTW; ++b) bcol[b] = (wave + NW * b) * 16 + l16;
+  for (int k = p - 2; k >= kstop; --k) {
+    const gptr<const double> Qk = glb(t.Q) + (size_t)k * t.qstride;
+    double acc[8][NTW];
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+#pragma unroll
+      for (int b = 0; b < NTW; ++b) acc[g][b] = 0.0;
+
+    // Two register buffers: the loads of k-step s + 1 are in flight while
+    // the MFMAs of step s run (and the other wave of the SIMD covers what
+    // is left of their latency).  With 8 | n every index of the
+    // carry part and of Q_k^T is in range, so only the X part of a
+    // workgroup whose rows reach the diagonal block (or the matrix end)
+    // needs masks: there loads are unconditional, from indexes clamped into
+    // the matrix, and the masks (bit g: row group g) are applied just before
+    // the MFMAs.  (A select right after a load, or a predicated load, makes
+    // the compiler wait for the loads early.)
+    double af[2][8], bf[2][NTW];
+    int msk[2];
+    auto step = [&](int u, auto masked_tag) {
+      constexpr bool MASKED = decltype(masked_tag)::value;
+      double a[8];
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        if constexpr (MASKED) a[g] = (msk[u] >> g & 1) ? af[u][g] : 0.0;
+        else a[g] = af[u][g];
+      }
+#pragma unroll
+      for (int b = 0; b < NTW; ++b)
+#pragma unroll
+        for (int g = 0; g < 8; ++g) acc[g][b] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[g], bf[u][b], acc[g][b], 0, 0, 0);
+    };
+    // n / 4 k-steps, an even number (8 | n): the loop body has no branch
+    // around its loads, so the wait before each step can leave the next
+    // step's loads in flight.
+    auto run_part = [&](auto load, auto masked_tag) {
+      load(0, 0);
+      int k0 = 0;
+      for (; k0 + 8 < n; k0 += 8) {
+        load(1, k0 + 4);
+        step(0, masked_tag);
+        load(0, k0 + 8);
+        step(1, masked_tag);
+      }
+      load(1, k0 + 4);
+      step(0, masked_tag);
+      step(1, masked_tag);
+    };
+    if (rbmin <= k) {
+      // [X_{r,k}] part: K rows 0..n-1 of Q_k^T
+      const gptr<const double> Xk = A + (size_t)k * n * lda;
+      if (R0 + MT <= N && (R0 + MT - 1) / n < k) {  // every row below block row k's diagonal: no masks
+        run_part(
+            [&](int u, int k0) {
+              const int kk = k0 + x;
+              const gptr<const double> xr = Xk + (size_t)kk * lda + R0 + y;
+#pragma unroll
+              for (int g = 0; g < 8; ++g) af[u][g] = xr[4 * g];
+#pragma unroll
+              for (int b = 0; b < NTW; ++b) bf[u][b] = Qk[(size_t)kk * n2 + bcol[b]];
+            },
+            std::false_type{});
+      } else {
+        run_part(
+            [&](int u, int k0) {
+              const int kk = k0 + x;
+              int mk = 0;
+#pragma unroll
+              for (int g = 0; g < 8; ++g) {
+                af[u][g] = Xk[(size_t)kk * lda + min(R0 + 4 * g + y, N - 1)];
+                mk |= (rblk[g] <= k && !(rblk[g] == k && rin[g] > kk)) << g;
+              }
+#pragma unroll
+              for (int b = 0; b < NTW; ++b) bf[u][b] = Qk[(size_t)kk * n2 + bcol[b]];
+              msk[u] = mk;
+            },
+            std::true_type{});
+      }
+    }
+    // [carry] part: K rows n..2n-1 of Q_k^T
+    {
+      const gptr<const double> Qc = Qk + (size_t)n * n2;
+      run_part(
+          [&](int u, int k0) {
+            const int kk = k0 + x;
+            const double* cr = carry + kk * ldcar + y * 8;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) af[u][g] = cr[g];
+#pragma unroll
+            for (int b = 0; b < NTW; ++b) bf[u][b] = Qc[(size_t)kk * n2 + bcol[b]];
+          },
+          std::false_type{});
+    }
+    __syncthreads();
+    // epilogue: lane holds rows 4 g + x, column tile * 16 + l16
+#pragma unroll
+    for (int b = 0; b < NTW; ++b) {
+      const int tile = wave + NW * b;
+      if (tile >= ntiles) continue;
+      const int col = tile * 16 + l16;
+      if (col >= n2) continue;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int i = 4 * g + x;
+        if (col < n) {
+          carry[cidx(col, i)] = acc[g][b];
+        } else {
+          const int row = R0 + i;
+          if (row < N && (!diag || row / n == k + 1)) A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[g][b];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // block column 0 (the carry after k = 0), for diag_only only its diagonal
+  if (!diag || rbmin == 0) {
+    for (int e = tid; e < n * MT; e += NT) {
+      const int i = e % MT, kk = e / MT;
+      const int row = R0 + i;
+      if (row < N && (!diag || row < n)) A[(size_t)kk * lda + row] = carry[cidx(kk, i)];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 rate calibration: independent chains of one instruction kind on
 // every CU (KIND 0: v_mfma_f64_16x16x4f64, 1: v_mfma_f64_4x4x4f64 with its
 // 4 blocks, 2: v_fma_f64), 8 chains per wave; the result is stored so

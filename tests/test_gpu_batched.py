@@ -217,8 +217,9 @@ def test_serving_alone_equals_mix(B, eps):
 
 def test_tuned_variants_match_oracle(B, eps):
     """Every (nb, scan_mti) pair the autotuner may pick gives the oracle's
-    inverse: the nb = 16 panel instances, the 64-row scan tiles, and the
-    16-row scan tiles below n = 256."""
+    inverse: the nb = 16 panel instances, the 64-row scan tiles, the 16-row
+    scan tiles below n = 256, and the 32-row 4x4x4 scan (scan_mti 9) where
+    8 divides n."""
     import torch
 
     from bsofi_amd import autotune
@@ -246,7 +247,8 @@ def test_tuned_variants_match_oracle(B, eps):
     with pytest.raises(ValueError):
         ctx.set_tuning(144, 0, 4)  # 64-row scan tiles need n <= 128
     ctx.clear_tuning()
-    assert ctx.get_tuning(64) == (32, 2)
+    assert ctx.get_tuning(64) == (32, 9)  # the 4x4x4 scan for 8 | n <= 256
+    assert ctx.get_tuning(36) == (32, 2)
     ctx.close()
 
 
@@ -296,12 +298,13 @@ def test_det_matches_slogdet(B):
 def test_diag_only_blocks_equal_full_inverse(B):
     """BSOFI_PHASE_DIAG_ONLY: the diagonal blocks G_ll equal, bit for bit,
     those of the full inverse -- for block sizes whose scan tiles straddle
-    block rows (n = 9, 36, 100), every scan tile height, and in a batch."""
+    block rows (n = 9, 36, 100, and n = 16 under the 32-row 4x4x4 scan),
+    every scan form, and in a batch."""
     import torch
 
-    cases = [(3, 5), (6, 4), (10, 3), (4, 7)]
+    cases = [(3, 5), (6, 4), (10, 3), (4, 7), (8, 3)]
     ctx = B.Context(0)
-    for mti in (0, 1, 2, 4):
+    for mti in (0, 1, 2, 4, 9):
         mats = []
         for k, (lx, p) in enumerate(cases):
             n = lx * lx
