@@ -117,6 +117,13 @@ class DeviceSet:
                                        self.p)
 
 
+def scan_kernel_name(n, scan_mti=0):
+    """The BSOI scan kernel a plan on block size n runs (the library's
+    default scan form unless scan_mti overrides it)."""
+    form = scan_mti or (9 if n <= 256 and n % 8 == 0 else (2 if n <= 256 else 1))
+    return "bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form == 9 else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)"
+
+
 def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None):
     import torch
 
@@ -447,7 +454,7 @@ def main():
         "resid_bound": bound,
         "resid_ok": bool(resid_max <= bound and all(i == 0 for i in r["infos"])),
         "roofline": {
-            "kernel": "bsoi_scan_kernel",
+            "kernel": scan_kernel_name(n, args.scan_mti),
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
