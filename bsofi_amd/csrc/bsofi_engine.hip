@@ -118,7 +118,11 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   // 32-wide sub-panel of 2n in (256, 512], else 4 waves of nb / 4 columns
   const int nw = (g.nb == 32 && m > 256 && m <= 512) ? 8 : 4;
   g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
-  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti : (n <= 256 ? (n % 8 == 0 ? kScan4 : 2) : 1);
+  // default scan form: the 4x4x4 scan from n = 128 (one workgroup per CU;
+  // below that its k-loops are short and batches of small matrices need the
+  // occupancy of the 16x16x4 form)
+  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti
+                                         : (n <= 256 ? (n >= 128 && n % 8 == 0 ? kScan4 : 2) : 1);
   g.scan_ntw = scan_ntw_for(n);
   return g;
 }

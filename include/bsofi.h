@@ -169,7 +169,7 @@ int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
  * 32 needs 2n <= 512, 16 needs 2n <= 1024), scan_mti = form of the BSOI
  * scan: 1, 2 or 4 = 16-row multiples per workgroup on the 16x16x4 FP64
  * MFMA (4 needs n <= 128, 2 needs n <= 256), 9 = 32 rows on the 4x4x4
- * FP64 MFMA (needs n <= 256 and n % 8 == 0; the default there).  0 keeps
+ * FP64 MFMA (needs n <= 256 and n % 8 == 0; the default for n >= 128).  0 keeps
  * the default; unsupported pairs return
  * BSOFI_ERR_UNSUPPORTED.  No counterpart in the reference, whose static
  * CPU/GPU split (KAPPA_*, *_SWITCH_*) these choices replace. */

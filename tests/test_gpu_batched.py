@@ -247,8 +247,8 @@ def test_tuned_variants_match_oracle(B, eps):
     with pytest.raises(ValueError):
         ctx.set_tuning(144, 0, 4)  # 64-row scan tiles need n <= 128
     ctx.clear_tuning()
-    assert ctx.get_tuning(64) == (32, 9)  # the 4x4x4 scan for 8 | n <= 256
-    assert ctx.get_tuning(36) == (32, 2)
+    assert ctx.get_tuning(64) == (32, 2)
+    assert ctx.get_tuning(144) == (32, 9)  # the 4x4x4 scan for 8 | n in [128, 256]
     ctx.close()
 
 
