@@ -27,7 +27,9 @@ from .workload import DeviceMatrices
 def candidates(n):
     nbs = [nb for nb, ok in ((32, 2 * n <= 512), (16, 2 * n <= 1024), (8, True)) if ok][:2]
     # scan forms: 16 * m rows on the 16x16x4 MFMA, or 9: 32 rows on the 4x4x4 MFMA
-    mtis = [m for m, ok in ((1, True), (2, n <= 256), (4, n <= 128), (9, n <= 256 and n % 8 == 0)) if ok]
+    # (from n = 128: below, its one workgroup per CU starves real batches,
+    # which the tuner's small trial batch does not show)
+    mtis = [m for m, ok in ((1, True), (2, n <= 256), (4, n <= 128), (9, 128 <= n <= 256 and n % 8 == 0)) if ok]
     return [(nb, m) for nb in nbs for m in mtis]
 
 

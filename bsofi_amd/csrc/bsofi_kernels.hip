@@ -520,10 +520,12 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   const int li = lane & 15, lk = lane >> 4;
   const int m = t.m, nbj = t.nbj;
 
-  __shared__ double Wp[NW][32][17];
   __shared__ double Wf[32][17];
   __shared__ double Tl[32][33];
-  __shared__ double Xt[NW][16][33];  // per-wave transpose tile
+  // per wave: the transpose tile X (16 x 33) in phases 1 and 3, the wave's
+  // share of W (32 x 17) between them
+  __shared__ double XW[NW][32 * 17];
+  auto Wp = [&](int w) { return reinterpret_cast<double (*)[17]>(XW[w]); };
 
   // the slab column of register r in the row-major-lanes layout
   gptr<double> cp[4];
@@ -561,7 +563,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     if constexpr (PREV) return vb[s][kq];
     else return vload(Vg, t.ldv, m, nbj, (wave + NW * s) * 16 + li, 4 * kq + lk);
   };
-  double (*X)[33] = Xt[wave];
+  double (*X)[33] = reinterpret_cast<double (*)[33]>(XW[wave]);
 
   // phase 1: W = V^T C, two independent accumulator pairs
   const bool two = nbj > 16;
@@ -597,10 +599,13 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     }
   }
   const d4 w0 = w0a + w0b, w1 = w1a + w1b;
+  {
+    double (*Wq)[17] = Wp(wave);  // the wave's X tile is done with
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    Wp[wave][crow(lk, r)][li] = w0[r];
-    Wp[wave][16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+    for (int r = 0; r < 4; ++r) {
+      Wq[crow(lk, r)][li] = w0[r];
+      Wq[16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+    }
   }
   __syncthreads();
   // phase 2: Wf = -op(T) W
@@ -609,7 +614,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     const int i = e >> 4, j = e & 15;
     double v = 0.0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) v += Wp[w][i][j];
+    for (int w = 0; w < NW; ++w) v += Wp(w)[i][j];
     W[i][j] = v;
   }
   __syncthreads();
