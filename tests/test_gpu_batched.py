@@ -219,7 +219,7 @@ def test_tuned_variants_match_oracle(B, eps):
     """Every (nb, scan_mti) pair the autotuner may pick gives the oracle's
     inverse: the nb = 16 panel instances, the 64-row scan tiles, the 16-row
     scan tiles below n = 256, and the 32-row 4x4x4 scan (scan_mti 9) where
-    8 divides n."""
+    8 divides n (from n = 128)."""
     import torch
 
     from bsofi_amd import autotune
@@ -298,11 +298,11 @@ def test_det_matches_slogdet(B):
 def test_diag_only_blocks_equal_full_inverse(B):
     """BSOFI_PHASE_DIAG_ONLY: the diagonal blocks G_ll equal, bit for bit,
     those of the full inverse -- for block sizes whose scan tiles straddle
-    block rows (n = 9, 36, 100, and n = 16 under the 32-row 4x4x4 scan),
-    every scan form, and in a batch."""
+    block rows (n = 9, 36, 100), every scan form (n = 144 under the 32-row
+    4x4x4 scan), and in a batch."""
     import torch
 
-    cases = [(3, 5), (6, 4), (10, 3), (4, 7), (8, 3)]
+    cases = [(3, 5), (6, 4), (10, 3), (4, 7), (8, 3), (12, 3)]
     ctx = B.Context(0)
     for mti in (0, 1, 2, 4, 9):
         mats = []

@@ -52,7 +52,9 @@ PHASE_CASES = [("dqmc", 2, 2), ("dqmc", 2, 3), ("dqmc", 2, 4), ("dqmc", 3, 5), (
                ("dqmc", 8, 6), ("dqmc", 10, 3), ("rand_eye", 1, 5), ("rand_eye", 17, 4), ("rand_full", 33, 3),
                ("dqmc", 12, 3), ("rand_eye", 130, 2),
                # 2n above 512: the m <= 1024 variants of the panel and reflector kernels
-               ("rand_eye", 300, 3), ("dqmc", 20, 2)]
+               ("rand_eye", 300, 3), ("dqmc", 20, 2),
+               # 8 | n in [128, 256]: the 4x4x4 scan by default
+               ("rand_eye", 128, 3), ("rand_full", 160, 2)]
 
 
 @pytest.mark.parametrize("kind,a,p", PHASE_CASES)
@@ -150,12 +152,15 @@ def test_c1_checksums(B, eps):
     assert oracle.residual(A0, A) <= 100 * N * eps
 
 
-def test_lda_larger_than_N(B, O, eps):
-    lx, p = 3, 4
+@pytest.mark.parametrize("lx,p,pad", [(3, 4, 5), (12, 3, 5), (12, 3, 6)])
+def test_lda_larger_than_N(B, O, eps, lx, p, pad):
+    """Padding rows are left alone (n = 144: the 4x4x4 scan; odd and even
+    lda)."""
     A0, m = dqmc.dqmc_matrix(lx, p)
     n = m["n"]
     N = n * p
-    lda = N + 5
+    lda = N + pad
+    print(f"case lda n={n} p={p} lda={lda}", flush=True)
     Ab = np.zeros((lda, N), order="F")
     Ab[:N] = A0
     Ab[N:] = 7.0  # padding rows must be left untouched
