@@ -1117,9 +1117,12 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   const int rbmin = R0 / n;
   const bool diag = t.diag_only != 0;
   const int kstop = diag ? max(rbmin - 1, 0) : 0;
+  // Q_k^T columns of this lane's tiles.  When the 16-column tiles do not
+  // divide evenly over the waves, a wave's last tiles lie beyond 2n: they
+  // read the last column (in bounds) and their results are never stored.
   int bcol[NTW];
 #pragma unroll
-  for (int b = 0; b < NTW; ++b) bcol[b] = (wave + NW * b) * 16 + l16;
+  for (int b = 0; b < NTW; ++b) bcol[b] = min((wave + NW * b) * 16 + l16, n2 - 1);
   for (int k = p - 2; k >= kstop; --k) {
     const gptr<const double> Qk = glb(t.Q) + (size_t)k * t.qstride;
     double acc[8][NTW];
@@ -1130,8 +1133,8 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
 
     // Two register buffers: the loads of k-step s + 1 are in flight while
     // the MFMAs of step s run (and the other wave of the SIMD covers what
-    // is left of their latency).  With 8 | n every index of the
-    // carry part and of Q_k^T is in range, so only the X part of a
+    // is left of their latency).  With 8 | n (and the clamped columns)
+    // every index of the carry part and of Q_k^T is in range, so only the X part of a
     // workgroup whose rows reach the diagonal block (or the matrix end)
     // needs masks: there loads are unconditional, from indexes clamped into
     // the matrix, and the masks (bit g: row group g) are applied just before
