@@ -90,6 +90,7 @@ def _load():
         "bsofi_plan_det": (i, [_vp, _dp, _dp]),
         "bsofi_det": (i, [_vp, _vp, i, i, i, _vp, _dp, _dp]),
         "bsofi_ctx_cached_bytes": (C.c_size_t, [_vp]),
+        "bsofi_ctx_set_lanes": (i, [_vp, i]),
         "bsofi_fp64_rate": (i, [_vp, i, _dp]),
         "bsofi_ctx_release_cache": (i, [_vp]),
         "bsofi_ctx_get_tuning": (i, [_vp, i, C.POINTER(i), C.POINTER(i)]),
@@ -256,6 +257,14 @@ class Context:
 
     def set_stream(self, stream):
         lib.bsofi_ctx_set_stream(self.h, stream)
+
+    def set_lanes(self, lanes):
+        """1 or 2 streams for batch plans created afterwards (default 2)."""
+        if lanes not in (1, 2):
+            raise ValueError("lanes must be 1 or 2")
+        rc = lib.bsofi_ctx_set_lanes(self.h, lanes)
+        if rc:
+            raise RuntimeError(f"bsofi_ctx_set_lanes failed ({rc})")
 
     def synchronize(self):
         rc = lib.bsofi_ctx_synchronize(self.h)

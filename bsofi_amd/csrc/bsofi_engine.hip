@@ -47,6 +47,7 @@ enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCA
 
 struct Launch {
   int family = 0, variant = 0, grid = 0, ntasks = 0, ldcar = 0;
+  int lane = 0;  // stream of a two-lane batch plan: 0 the context's, 1 its side stream
   size_t task_off = 0, item_off = 0, shmem = 0;
 };
 
@@ -165,6 +166,8 @@ struct bsofi_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipStream_t side = nullptr;  // second lane of batch plans, created on first use
+  int lanes = 2;               // lanes of batch plans created on the context
   std::map<int, bsofi::Tune> tune;  // per block size n
   // Device buffers of destroyed plans (task blobs, workspaces), reused by
   // later plans on this context so that steady-state plan churn calls
@@ -247,6 +250,8 @@ struct bsofi_plan_s {
   bool has_det = false;
   int count = 0;
   bool use_graph = false;
+  int lanes = 1;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int timed_family = -1;
@@ -279,7 +284,14 @@ static double* Tptr(const MatJob& m, int k, int j) {
 struct Planner {
   Blob& blob;
   std::vector<Launch>& out;
+  int lane = 0;
   Planner(Blob& b, std::vector<Launch>& o) : blob(b), out(o) {}
+
+  // every launch of the planner goes through here, tagged with its lane
+  void push(Launch L) {
+    L.lane = lane;
+    out.push_back(L);
+  }
 
   template <class T>
   void emit(int family, int variant, const std::vector<T>& tasks, const std::vector<Item>& items, int ldcar = 0,
@@ -294,7 +306,7 @@ struct Planner {
     L.grid = (int)items.size();
     L.ldcar = ldcar;
     L.shmem = shmem;
-    out.push_back(L);
+    push(L);
   }
 
   // refl_apply_kernel instance for the largest m: 4 waves with 1, 2 or 4
@@ -309,6 +321,20 @@ struct Planner {
     }
     return mm <= 512 ? 104 : 108;
   }
+
+  // Reflector tasks of one launch step, one workgroup per 16-column slab.
+  // The kernel instance follows from the largest m of the step; every
+  // instance sums a task's partial products in the same order, so a task's
+  // bits do not depend on the other matrices of the launch.
+  struct ReflRound {
+    std::vector<ReflTask> t;
+    std::vector<Item> i;
+    void add(const ReflTask& q, int slabs) {
+      for (int s = 0; s < slabs; ++s) i.push_back({(int)t.size(), s});
+      t.push_back(q);
+    }
+  };
+  void emit_refl(const ReflRound& R) { emit(F_REFL, refl_variant(R.t), R.t, R.i); }
 
   // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------
   void bsof(std::vector<MatJob>& mats) {
@@ -326,8 +352,7 @@ struct Planner {
     for (size_t r = 0; r < rounds; ++r) {
       std::map<int, std::vector<PanelTask>> pt;
       std::map<int, std::vector<Item>> pi;
-      std::vector<ReflTask> rt;
-      std::vector<Item> ri;
+      ReflRound rr;
       for (size_t i = 0; i < mats.size(); ++i) {
         if (r >= seq[i].size()) continue;
         const MatJob& M = mats[i];
@@ -360,13 +385,11 @@ struct Planner {
           q.w1 = w1;
           q.w2 = w2;
           q.trans = 1;
-          const int slabs = (w1 + w2 + 15) / 16;
-          for (int s = 0; s < slabs; ++s) ri.push_back({(int)rt.size(), s});
-          rt.push_back(q);
+          rr.add(q, (w1 + w2 + 15) / 16);
         }
       }
       for (auto& kv : pt) emit(F_PANEL, kv.first, kv.second, pi[kv.first]);
-      emit(F_REFL, refl_variant(rt), rt, ri);
+      emit_refl(rr);
     }
   }
 
@@ -454,7 +477,7 @@ struct Planner {
       L.task_off = blob.push(tt);
       L.ntasks = (int)tt.size();
       L.grid = (int)tt.size();
-      out.push_back(L);
+      push(L);
     }
     for (int J = 1; J < maxJ; ++J) {
       GemmBatch b1, b2;
@@ -545,11 +568,10 @@ struct Planner {
       L.task_off = blob.push(et);
       L.ntasks = (int)et.size();
       L.grid = 1024;
-      out.push_back(L);
+      push(L);
     }
     for (int r = 0; r < maxS; ++r) {
-      std::vector<ReflTask> rt;
-      std::vector<Item> ri;
+      ReflRound rr;
       for (auto& M : mats) {
         const Geo& g = M.g;
         for (int k = 0; k <= g.p - 2; ++k) {
@@ -573,12 +595,10 @@ struct Planner {
           q.w1 = n2 - j * g.nb;
           q.w2 = 0;
           q.trans = 0;
-          const int slabs = (q.w1 + 15) / 16;
-          for (int s = 0; s < slabs; ++s) ri.push_back({(int)rt.size(), s});
-          rt.push_back(q);
+          rr.add(q, (q.w1 + 15) / 16);
         }
       }
-      emit(F_REFL, refl_variant(rt), rt, ri);
+      emit_refl(rr);
     }
     // row scans, grouped by kernel instance; heaviest tiles first
     struct Var { int mti, ntw; };
@@ -837,6 +857,7 @@ int bsofi_ctx_destroy(bsofi_ctx_t c) {
   (void)hipStreamSynchronize(c->stream);
   ctx_trim(c);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return 0;
 }
@@ -855,6 +876,12 @@ int bsofi_ctx_set_stream(bsofi_ctx_t c, void* s) {
 }
 
 size_t bsofi_ctx_cached_bytes(bsofi_ctx_t c) { return c ? c->cache_bytes : 0; }
+
+int bsofi_ctx_set_lanes(bsofi_ctx_t c, int lanes) {
+  if (!c || (lanes != 1 && lanes != 2)) return -1;
+  c->lanes = lanes;
+  return 0;
+}
 
 int bsofi_ctx_release_cache(bsofi_ctx_t c) {
   if (!c) return -1;
@@ -951,12 +978,55 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     L.grid = 1;
     P->launches.push_back(L);
   }
-  if (phases & BSOFI_PHASE_BSOF) {
-    pl.bsof(jobs);
-    pl.det(jobs);
+  // A batch runs in two lanes: the matrices are split into two groups of
+  // about equal work (longest first, each to the lighter lane), and each
+  // group's launches go to a stream of its own, so that one lane's
+  // latency-bound launches (panel QRs, short reflector steps, the tails of
+  // wide launches) run beside the other lane's work.  A matrix's launches
+  // stay in one lane and in order, and its kernels and tasks are the ones it
+  // gets in any other grouping, so its result is the same bits.
+  P->lanes = count >= 2 ? ctx->lanes : 1;
+  std::vector<int> lane_of(count, 0);
+  if (P->lanes == 2) {
+    auto work = [&](int i) {
+      const double n = jobs[i].g.n, p = jobs[i].g.p;
+      return n * n * n * p * (p + 4.0);  // scan ~ n^3 p^2, BSOF and BSTRI ~ n^3 p
+    };
+    std::vector<int> ord(count);
+    for (int i = 0; i < count; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work(a) > work(b); });
+    double load[2] = {0.0, 0.0};
+    for (int i : ord) {
+      const int l = load[1] < load[0] ? 1 : 0;
+      lane_of[i] = l;
+      load[l] += work(i);
+    }
   }
-  if (phases & BSOFI_PHASE_BSTRI) pl.bstri(jobs);
-  if (phases & BSOFI_PHASE_BSOI) pl.bsoi(jobs, max_shmem, (phases & BSOFI_PHASE_DIAG_ONLY) != 0);
+  for (int l = 0; l < P->lanes; ++l) {
+    std::vector<MatJob> grp;
+    for (int i = 0; i < count; ++i)
+      if (lane_of[i] == l) grp.push_back(jobs[i]);
+    if (grp.empty()) continue;
+    pl.lane = l;
+    if (phases & BSOFI_PHASE_BSOF) {
+      pl.bsof(grp);
+      pl.det(grp);
+    }
+    if (phases & BSOFI_PHASE_BSTRI) pl.bstri(grp);
+    if (phases & BSOFI_PHASE_BSOI) pl.bsoi(grp, max_shmem, (phases & BSOFI_PHASE_DIAG_ONLY) != 0);
+  }
+  if (P->lanes == 2) {
+    bool ok = ctx->side || hip_ok(hipStreamCreate(&ctx->side), "hipStreamCreate(side)");
+    ok = ok && hip_ok(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming), "hipEventCreate");
+    ok = ok && hip_ok(hipEventCreateWithFlags(&P->join, hipEventDisableTiming), "hipEventCreate");
+    if (!ok) {
+      if (P->fork) (void)hipEventDestroy(P->fork);
+      if (P->join) (void)hipEventDestroy(P->join);
+      if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
+      delete P;
+      return BSOFI_ERR_NO_DEVICE;
+    }
+  }
   for (auto& L : P->launches) {
     if (L.family == F_SCAN) {
       const int variant = L.variant;
@@ -983,14 +1053,36 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   return 0;
 }
 
+// Issues the plan's launches.  A two-lane plan forks the side stream off the
+// context's stream right after the info-word fill (an event the side stream
+// waits for) and joins it back at the end, so that work the caller queues on
+// the context's stream afterwards sees both lanes done; the two lanes'
+// launches are issued alternately, so that neither waits on the host for the
+// other's.  Under stream capture the same calls record a graph with two
+// branches.
 static void run_all(bsofi_plan_s* P) {
-  hipStream_t s = P->ctx->stream;
+  hipStream_t st[2] = {P->ctx->stream, P->lanes == 2 ? P->ctx->side : P->ctx->stream};
+  std::vector<const Launch*> lane[2];
   for (const Launch& L : P->launches) {
     if (L.family == F_FILL) {
-      hipLaunchKernelGGL(fill_int_kernel, dim3((L.ntasks + 255) / 256), dim3(256), 0, s, P->d_info, L.ntasks,
+      hipLaunchKernelGGL(fill_int_kernel, dim3((L.ntasks + 255) / 256), dim3(256), 0, st[0], P->d_info, L.ntasks,
                          INT_MAX);
       continue;
     }
+    lane[L.lane & 1].push_back(&L);
+  }
+  if (P->lanes == 2) {
+    (void)hipEventRecord(P->fork, st[0]);
+    (void)hipStreamWaitEvent(st[1], P->fork, 0);
+  }
+  std::vector<const Launch*> order;
+  order.reserve(lane[0].size() + lane[1].size());
+  for (size_t i = 0; i < std::max(lane[0].size(), lane[1].size()); ++i)
+    for (int l = 0; l < 2; ++l)
+      if (i < lane[l].size()) order.push_back(lane[l][i]);
+  for (const Launch* Lp : order) {
+    const Launch& L = *Lp;
+    hipStream_t s = st[L.lane & 1];
     const bool timed = (L.family == P->timed_family);
     if (timed) {
       if ((size_t)P->ev_used + 2 > P->ev.size()) {
@@ -1007,6 +1099,10 @@ static void run_all(bsofi_plan_s* P) {
       (void)hipEventRecord(P->ev[P->ev_used + 1], s);
       P->ev_used += 2;
     }
+  }
+  if (P->lanes == 2) {
+    (void)hipEventRecord(P->join, st[1]);
+    (void)hipStreamWaitEvent(st[0], P->join, 0);
   }
 }
 
@@ -1117,6 +1213,8 @@ int bsofi_plan_destroy(bsofi_plan_t P) {
   (void)hipStreamSynchronize(P->ctx->stream);
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->fork) (void)hipEventDestroy(P->fork);
+  if (P->join) (void)hipEventDestroy(P->join);
   for (auto e : P->ev) (void)hipEventDestroy(e);
   // the stream has drained: the buffers go back to the context for reuse
   ctx_free(P->ctx, P->d_blob, P->blob_bytes);

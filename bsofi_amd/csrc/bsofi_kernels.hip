@@ -565,7 +565,13 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   };
   double (*X)[33] = reinterpret_cast<double (*)[33]>(XW[wave]);
 
-  // phase 1: W = V^T C, two independent accumulator pairs
+  // phase 1: W = V^T C.  The partial sums follow one order for both wave
+  // counts, so that a task gets the same bits from either kernel: row tile t
+  // goes to chain (t mod 4, (t / 4) mod 2), each chain adds its tiles in
+  // increasing t, and W = sum over q = 0..3 of (chain (q,0) + chain (q,1)).
+  // With 4 waves, wave q holds both chains of q (accumulators a and b, by
+  // the parity of s); with 8 waves, wave w holds the one chain
+  // (w mod 4, w / 4), and phase 2 adds waves q and q + 4 first.
   const bool two = nbj > 16;
   d4 acc[RT];
   d4 w0a = d4zero(), w0b = d4zero(), w1a = d4zero(), w1b = d4zero();
@@ -588,7 +594,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     if ((wave + NW * s) * 16 < m) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        if (s & 1) {
+        if (NW == 4 && (s & 1)) {
           w0b = mfma(va[r][0], acc[s][r], w0b);
           if (two) w1b = mfma(va[r][1], acc[s][r], w1b);
         } else {
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
       }
     }
   }
-  const d4 w0 = w0a + w0b, w1 = w1a + w1b;
+  const d4 w0 = NW == 8 ? w0a : w0a + w0b, w1 = NW == 8 ? w1a : w1a + w1b;
   {
     double (*Wq)[17] = Wp(wave);  // the wave's X tile is done with
 #pragma unroll
@@ -613,8 +619,13 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   for (int e = tid; e < 32 * 16; e += NT) {
     const int i = e >> 4, j = e & 15;
     double v = 0.0;
+    if constexpr (NW == 8) {
 #pragma unroll
-    for (int w = 0; w < NW; ++w) v += Wp(w)[i][j];
+      for (int q = 0; q < 4; ++q) v += Wp(q)[i][j] + Wp(q + 4)[i][j];
+    } else {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) v += Wp(q)[i][j];
+    }
     W[i][j] = v;
   }
   __syncthreads();

@@ -176,6 +176,15 @@ int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
 int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti);
 int bsofi_ctx_get_tuning(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti);
 int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
+/* Lanes of batch plans created on the context afterwards: 2 (the default)
+ * splits a plan of two or more matrices into two groups of about equal work
+ * that run on two streams (the context's and a second one it creates),
+ * forked after the plan's first launch and joined before it returns to the
+ * context's stream; 1 keeps every launch on the context's stream.  Results
+ * are the same bits either way.  Callers that already run several plans at
+ * once on several contexts (the serving loop) want 1.  Returns -1 for other
+ * values.  No counterpart in the reference. */
+int bsofi_ctx_set_lanes(bsofi_ctx_t ctx, int lanes);
 /* Destroyed plans return their device buffers (task tables, workspace) to
  * the context, and later plans on it reuse them, so that creating and
  * destroying plans in a loop allocates nothing once warm.  These report and

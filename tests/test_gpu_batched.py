@@ -44,7 +44,9 @@ def from_dev(dA, N):
     return dA.cpu().numpy().T[:N, :N]
 
 
-MIX = [(2, 3), (3, 7), (6, 5), (8, 4), (4, 12), (10, 3), (1, 9)]  # (lattice side, p)
+# (lattice side, p): n from 1 to 144, so that reflector steps of m <= 256
+# (4-wave kernel) and m > 256 (8-wave kernel) share rounds
+MIX = [(2, 3), (3, 7), (6, 5), (8, 4), (4, 12), (10, 3), (1, 9), (12, 3)]
 
 
 def test_batched_mix_equals_alone_and_oracle(B, eps):
@@ -76,6 +78,61 @@ def test_batched_mix_equals_alone_and_oracle(B, eps):
         assert rel(alone, G) <= 1e-10
         assert oracle.residual(A0, alone) <= 100 * n * p * eps
     ctx.close()
+
+
+def test_many_slab_reflector_launch_same_bits(B, eps):
+    """24 matrices of n = 144 in one plan: each BSOF reflector launch holds
+    600 slabs of m = 288 rows (more workgroups than CUs, matrices in both
+    lanes of the plan), against 25 slabs alone.  The bits must agree."""
+    import torch
+
+    ctx = B.Context(0)
+    cases = [dqmc.dqmc_matrix(12, 3, instance=k)[0] for k in range(24)]
+    n, p = 144, 3
+    devs = [to_dev(A0) for A0 in cases]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for _ in cases]
+    torch.cuda.synchronize()
+    info = B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t in zip(devs, taus)])
+    assert info == [0] * len(cases)
+    for k in (0, 23):
+        print(f"many-slab case {k}: n={n} p={p}", flush=True)
+        d = to_dev(cases[k])
+        t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+        alone = from_dev(d, n * p)
+        assert np.array_equal(alone, from_dev(devs[k], n * p)), f"case {k} differs between alone and mix"
+        assert oracle.residual(cases[k], alone) <= 100 * n * p * eps
+    ctx.close()
+
+
+def test_one_lane_same_bits(B):
+    """A batch plan in one lane (every launch on the context's stream) and in
+    two (the default) gives the same bits, eagerly and as a graph."""
+    import torch
+
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=k)[0] for k, (lx, p) in enumerate(MIX)]
+    sizes = [(lx * lx, p) for lx, p in MIX]
+    out = {}
+    for lanes in (1, 2):
+        for graph in (False, True):
+            ctx = B.Context(0)
+            ctx.set_lanes(lanes)
+            devs = [to_dev(A0) for A0 in inputs]
+            taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in sizes]
+            torch.cuda.synchronize()
+            plan = B.Plan(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t, (n, p) in zip(devs, taus, sizes)])
+            plan.use_graph(graph)
+            print(f"lanes={lanes} graph={graph}", flush=True)
+            plan.execute()
+            ctx.synchronize()
+            assert plan.info() == [0] * len(MIX)
+            plan.close()
+            out[lanes, graph] = [from_dev(d, n * p).copy() for d, (n, p) in zip(devs, sizes)]
+            ctx.close()
+    for key in out:
+        for k in range(len(MIX)):
+            assert np.array_equal(out[key][k], out[1, False][k]), f"case {k} differs with lanes/graph {key}"
 
 
 def test_graph_replays_match_eager(B):
