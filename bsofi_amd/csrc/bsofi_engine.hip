@@ -48,6 +48,7 @@ enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCA
 struct Launch {
   int family = 0, variant = 0, grid = 0, ntasks = 0, ldcar = 0;
   int lane = 0;  // stream of a two-lane batch plan: 0 the context's, 1 its side stream
+  int spw = 1;   // reflector launches: slabs per workgroup
   size_t task_off = 0, item_off = 0, shmem = 0;
 };
 
@@ -285,6 +286,7 @@ struct Planner {
   Blob& blob;
   std::vector<Launch>& out;
   int lane = 0;
+  size_t cus = 256;  // compute units of the device
   Planner(Blob& b, std::vector<Launch>& o) : blob(b), out(o) {}
 
   // every launch of the planner goes through here, tagged with its lane
@@ -322,19 +324,37 @@ struct Planner {
     return mm <= 512 ? 104 : 108;
   }
 
-  // Reflector tasks of one launch step, one workgroup per 16-column slab.
-  // The kernel instance follows from the largest m of the step; every
-  // instance sums a task's partial products in the same order, so a task's
-  // bits do not depend on the other matrices of the launch.
+  // Reflector tasks of one launch step.  The kernel instance follows from
+  // the largest m of the step; every instance sums a task's partial
+  // products in the same order, so a task's bits do not depend on the other
+  // matrices of the launch.  In the 8-wave, 4-tile instance a workgroup
+  // takes spw consecutive 16-column slabs of one task (variant 204): 1 while
+  // the slabs fit one round of workgroups on the GPU (each workgroup is
+  // latency-bound, so more slabs per workgroup would only lengthen the
+  // launch), else as many as it takes to fit one round, at most 8, so that V
+  // is loaded once per workgroup and each slab's loads overlap the previous
+  // slab's work.
   struct ReflRound {
     std::vector<ReflTask> t;
-    std::vector<Item> i;
-    void add(const ReflTask& q, int slabs) {
-      for (int s = 0; s < slabs; ++s) i.push_back({(int)t.size(), s});
+    std::vector<int> slabs;
+    size_t total = 0;
+    void add(const ReflTask& q, int n) {
       t.push_back(q);
+      slabs.push_back(n);
+      total += (size_t)n;
     }
   };
-  void emit_refl(const ReflRound& R) { emit(F_REFL, refl_variant(R.t), R.t, R.i); }
+  void emit_refl(const ReflRound& R) {
+    int variant = refl_variant(R.t);
+    const int spw = variant == 104 ? (int)std::min<size_t>(8, (R.total + cus - 1) / cus) : 1;
+    if (spw > 1) variant = 204;
+    std::vector<Item> items;
+    for (size_t k = 0; k < R.t.size(); ++k)
+      for (int s = 0; s < R.slabs[k]; s += spw) items.push_back({(int)k, s});
+    const size_t before = out.size();
+    emit(F_REFL, variant, R.t, items);
+    if (out.size() > before) out.back().spw = std::max(spw, 1);
+  }
 
   // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------
   void bsof(std::vector<MatJob>& mats) {
@@ -734,11 +754,14 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       auto t = (const ReflTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
       switch (L.variant) {
-        case 1: hipLaunchKernelGGL((refl_apply_kernel<1, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 2: hipLaunchKernelGGL((refl_apply_kernel<2, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 4: hipLaunchKernelGGL((refl_apply_kernel<4, 4>), dim3(L.grid), dim3(256), 0, s, t, it); break;
-        case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it); break;
-        default: hipLaunchKernelGGL((refl_apply_kernel<8, 8>), dim3(L.grid), dim3(512), 0, s, t, it); break;
+        case 1: hipLaunchKernelGGL((refl_apply_kernel<1, 4>), dim3(L.grid), dim3(256), 0, s, t, it, L.spw); break;
+        case 2: hipLaunchKernelGGL((refl_apply_kernel<2, 4>), dim3(L.grid), dim3(256), 0, s, t, it, L.spw); break;
+        case 4: hipLaunchKernelGGL((refl_apply_kernel<4, 4>), dim3(L.grid), dim3(256), 0, s, t, it, L.spw); break;
+        case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
+        case 204:
+          hipLaunchKernelGGL((refl_apply_kernel<4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
+          break;
+        default: hipLaunchKernelGGL((refl_apply_kernel<8, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
       }
       break;
     }
@@ -969,6 +992,9 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   int max_shmem = 65536;
   (void)hipDeviceGetAttribute(&max_shmem, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
   Planner pl(P->blob, P->launches);
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
+    pl.cus = (size_t)cus;
   if (count > 0) {
     // info words start at INT_MAX ("no zero pivot seen"); the fill launch
     // addresses the workspace directly (task_off holds no blob offset).

@@ -497,28 +497,45 @@ __device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, in
   return V[(size_t)col * ldv + row];
 }
 
-// The target slab lives in MFMA accumulator registers for the whole kernel:
-// NW waves (4 or 8); wave w owns row tiles w, w+NW, ... (RT of them,
-// m <= 16 NW RT).  Eight waves halve the loads each lane has in flight at
-// entry (a wave keeps at most 63 outstanding) and put two waves per SIMD.  The C layout
-// of a 16 x 16 accumulator tile is exactly four consecutive B fragments of
-// a K = 16 block (register r = rows 4r..4r+3), so phase 1 multiplies
-// straight out of the accumulators.
+// Workgroup barrier that orders LDS only: waits for the wave's LDS
+// operations, not for its global loads, so that loads issued before it stay
+// in flight across it (__syncthreads also waits for every global access).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The target slab lives in MFMA accumulator registers: NW waves (4 or 8);
+// wave w owns row tiles w, w+NW, ... (RT of them, m <= 16 NW RT).  Eight
+// waves halve the loads each lane has in flight at entry (a wave keeps at
+// most 63 outstanding) and put two waves per SIMD.  The C layout of a
+// 16 x 16 accumulator tile is exactly four consecutive B fragments of a
+// K = 16 block (register r = rows 4r..4r+3), so phase 1 multiplies straight
+// out of the accumulators.
 // Global memory is only touched in "row-major lanes" layouts where the 16
 // lanes of a quarter-wave read 16 consecutive rows of one column (128-byte
 // segments): the slab as C[rt*16 + li][4r + lk] and V as
 // V[rt*16 + li][4 kq + lk] (phase 3's A fragments).  A per-wave LDS tile
 // turns them into the accumulator layout and phase 1's V^T fragments, and
-// turns the result back for the store.  All loads are issued up front.
-template <int RT, int NW>
+// turns the result back for the store.
+// MS: a workgroup takes slabs it.sub .. it.sub + spw - 1 of its task, one
+// after the other: V (in registers) and T (in LDS) are loaded once, and the
+// next slab's C is loaded while the current one goes through phases 2 and 3
+// and is stored (the barriers order LDS only).  Without MS a workgroup takes
+// the one slab it.sub.  Every slab goes through the same arithmetic either
+// way.
+template <int RT, int NW, bool MS = false>
 __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
-                                                             const Item* __restrict__ items) {
+                                                             const Item* __restrict__ items, int spw) {
+  static_assert(!MS || RT <= 4, "the multi-slab form holds V in registers");
   constexpr int NT = 64 * NW;
   const Item it = items[blockIdx.x];
   const ReflTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int m = t.m, nbj = t.nbj;
+  const int sub_end = MS ? min(it.sub + spw, (t.w1 + t.w2 + 15) / 16) : it.sub + 1;
 
   __shared__ double Wf[32][17];
   __shared__ double Tl[32][33];
@@ -528,17 +545,20 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   auto Wp = [&](int w) { return reinterpret_cast<double (*)[17]>(XW[w]); };
 
   // the slab column of register r in the row-major-lanes layout
-  gptr<double> cp[4];
-  const gptr<const double> Vg = glb(t.V);
+  auto columns = [&](int sub, gptr<double>(&cp)[4]) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int vcol = it.sub * 16 + 4 * r + lk;
-    cp[r] = nullptr;
-    if (vcol < t.w1)
-      cp[r] = glb(t.C1) + (size_t)vcol * t.ldc;
-    else if (vcol - t.w1 < t.w2)
-      cp[r] = glb(t.C2) + (size_t)(vcol - t.w1) * t.ldc;
-  }
+    for (int r = 0; r < 4; ++r) {
+      const int vcol = sub * 16 + 4 * r + lk;
+      cp[r] = nullptr;
+      if (vcol < t.w1)
+        cp[r] = glb(t.C1) + (size_t)vcol * t.ldc;
+      else if (vcol - t.w1 < t.w2)
+        cp[r] = glb(t.C2) + (size_t)(vcol - t.w1) * t.ldc;
+    }
+  };
+  gptr<double> cp[4];
+  columns(it.sub, cp);
+  const gptr<const double> Vg = glb(t.V);
   for (int e = tid; e < 32 * 32; e += NT) {
     const int i = e & 31, j = e >> 5;
     Tl[i][j] = (i < nbj && j < nbj) ? glb(t.T)[(size_t)j * t.ldt + i] : 0.0;
@@ -549,12 +569,19 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   constexpr bool PREV = RT <= 4;
   double cl[RT][4];              // C[rt*16 + li][4r + lk]
   double vb[PREV ? RT : 1][8];   // V[rt*16 + li][4kq + lk]
+  auto load_c = [&](const gptr<double>(&c)[4]) {
 #pragma unroll
-  for (int s = 0; s < RT; ++s) {
-    const int row = (wave + NW * s) * 16 + li;
+    for (int s = 0; s < RT; ++s) {
+      const int row = (wave + NW * s) * 16 + li;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) cl[s][r] = (cp[r] && row < m) ? cp[r][row] : 0.0;
-    if constexpr (PREV) {
+      for (int r = 0; r < 4; ++r) cl[s][r] = (c[r] && row < m) ? c[r][row] : 0.0;
+    }
+  };
+  load_c(cp);
+  if constexpr (PREV) {
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      const int row = (wave + NW * s) * 16 + li;
 #pragma unroll
       for (int kq = 0; kq < 8; ++kq) vb[s][kq] = vload(Vg, t.ldv, m, nbj, row, 4 * kq + lk);
     }
@@ -564,101 +591,117 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
     else return vload(Vg, t.ldv, m, nbj, (wave + NW * s) * 16 + li, 4 * kq + lk);
   };
   double (*X)[33] = reinterpret_cast<double (*)[33]>(XW[wave]);
-
-  // phase 1: W = V^T C.  The partial sums follow one order for both wave
-  // counts, so that a task gets the same bits from either kernel: row tile t
-  // goes to chain (t mod 4, (t / 4) mod 2), each chain adds its tiles in
-  // increasing t, and W = sum over q = 0..3 of (chain (q,0) + chain (q,1)).
-  // With 4 waves, wave q holds both chains of q (accumulators a and b, by
-  // the parity of s); with 8 waves, wave w holds the one chain
-  // (w mod 4, w / 4), and phase 2 adds waves q and q + 4 first.
   const bool two = nbj > 16;
-  d4 acc[RT];
-  d4 w0a = d4zero(), w0b = d4zero(), w1a = d4zero(), w1b = d4zero();
+  __shared__ double W[32][17];
+
+  for (int sub = it.sub; sub < sub_end; ++sub) {
+    // phase 1: W = V^T C.  The partial sums follow one order for both wave
+    // counts, so that a task gets the same bits from either kernel: row tile
+    // t goes to chain (t mod 4, (t / 4) mod 2), each chain adds its tiles in
+    // increasing t, and W = sum over q = 0..3 of (chain (q,0) + chain (q,1)).
+    // With 4 waves, wave q holds both chains of q (accumulators a and b, by
+    // the parity of s); with 8 waves, wave w holds the one chain
+    // (w mod 4, w / 4), and phase 2 adds waves q and q + 4 first.
+    d4 acc[RT];
 #pragma unroll
-  for (int s = 0; s < RT; ++s) {
-    // slab tile -> accumulator layout C[4r + lk][li]
+    for (int s = 0; s < RT; ++s) {
+      // slab tile -> accumulator layout C[4r + lk][li]
 #pragma unroll
-    for (int r = 0; r < 4; ++r) X[li][4 * r + lk] = cl[s][r];
+      for (int r = 0; r < 4; ++r) X[li][4 * r + lk] = cl[s][r];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[s][r] = X[4 * r + lk][li];
-    // V tile -> V^T fragments V[4r + lk][li], V[4r + lk][16 + li]
-#pragma unroll
-    for (int kq = 0; kq < 8; ++kq) X[li][4 * kq + lk] = vtile(s, kq);
-    double va[4][2];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      va[r][0] = X[4 * r + lk][li];
-      va[r][1] = X[4 * r + lk][16 + li];
+      for (int r = 0; r < 4; ++r) acc[s][r] = X[4 * r + lk][li];
     }
-    if ((wave + NW * s) * 16 < m) {
+    d4 w0a = d4zero(), w0b = d4zero(), w1a = d4zero(), w1b = d4zero();
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      // V tile -> V^T fragments V[4r + lk][li], V[4r + lk][16 + li]
+#pragma unroll
+      for (int kq = 0; kq < 8; ++kq) X[li][4 * kq + lk] = vtile(s, kq);
+      double va[4][2];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        if (NW == 4 && (s & 1)) {
-          w0b = mfma(va[r][0], acc[s][r], w0b);
-          if (two) w1b = mfma(va[r][1], acc[s][r], w1b);
-        } else {
-          w0a = mfma(va[r][0], acc[s][r], w0a);
-          if (two) w1a = mfma(va[r][1], acc[s][r], w1a);
+        va[r][0] = X[4 * r + lk][li];
+        va[r][1] = X[4 * r + lk][16 + li];
+      }
+      if ((wave + NW * s) * 16 < m) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (NW == 4 && (s & 1)) {
+            w0b = mfma(va[r][0], acc[s][r], w0b);
+            if (two) w1b = mfma(va[r][1], acc[s][r], w1b);
+          } else {
+            w0a = mfma(va[r][0], acc[s][r], w0a);
+            if (two) w1a = mfma(va[r][1], acc[s][r], w1a);
+          }
         }
       }
     }
-  }
-  const d4 w0 = NW == 8 ? w0a : w0a + w0b, w1 = NW == 8 ? w1a : w1a + w1b;
-  {
-    double (*Wq)[17] = Wp(wave);  // the wave's X tile is done with
+    const d4 w0 = NW == 8 ? w0a : w0a + w0b, w1 = NW == 8 ? w1a : w1a + w1b;
+    {
+      double (*Wq)[17] = Wp(wave);  // the wave's X tile is done with
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      Wq[crow(lk, r)][li] = w0[r];
-      Wq[16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+      for (int r = 0; r < 4; ++r) {
+        Wq[crow(lk, r)][li] = w0[r];
+        Wq[16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+      }
     }
-  }
-  __syncthreads();
-  // phase 2: Wf = -op(T) W
-  __shared__ double W[32][17];
-  for (int e = tid; e < 32 * 16; e += NT) {
-    const int i = e >> 4, j = e & 15;
-    double v = 0.0;
-    if constexpr (NW == 8) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v += Wp(q)[i][j] + Wp(q + 4)[i][j];
-    } else {
-#pragma unroll
-      for (int q = 0; q < NW; ++q) v += Wp(q)[i][j];
+    // the next slab's C is in flight from here to its phase 1
+    gptr<double> cn[4];
+    const bool more = MS && sub + 1 < sub_end;
+    if (more) {
+      columns(sub + 1, cn);
+      load_c(cn);
     }
-    W[i][j] = v;
-  }
-  __syncthreads();
-  // op(T) W on MFMA: waves 0 and 1 form rows 0-15 and 16-31 (K = 32).
-  // Tl is zero outside the nbj x nbj triangle, so no masking is needed.
-  if (wave < 2) {
-    const int i = wave * 16 + li;
-    d4 tw = d4zero();
+    lds_barrier();
+    // phase 2: Wf = -op(T) W
+    for (int e = tid; e < 32 * 16; e += NT) {
+      const int i = e >> 4, j = e & 15;
+      double v = 0.0;
+      if constexpr (NW == 8) {
 #pragma unroll
-    for (int kc = 0; kc < 8; ++kc) {
-      const int k = 4 * kc + lk;
-      tw = mfma(t.trans ? Tl[k][i] : Tl[i][k], W[k][li], tw);
+        for (int q = 0; q < 4; ++q) v += Wp(q)[i][j] + Wp(q + 4)[i][j];
+      } else {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) v += Wp(q)[i][j];
+      }
+      W[i][j] = v;
     }
+    lds_barrier();
+    // op(T) W on MFMA: waves 0 and 1 form rows 0-15 and 16-31 (K = 32).
+    // Tl is zero outside the nbj x nbj triangle, so no masking is needed.
+    if (wave < 2) {
+      const int i = wave * 16 + li;
+      d4 tw = d4zero();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
-  }
-  __syncthreads();
-  // phase 3: C += V Wf, then back to the row-major-lanes layout and out
+      for (int kc = 0; kc < 8; ++kc) {
+        const int k = 4 * kc + lk;
+        tw = mfma(t.trans ? Tl[k][i] : Tl[i][k], W[k][li], tw);
+      }
 #pragma unroll
-  for (int s = 0; s < RT; ++s) {
-    const int rt = wave + NW * s;
-    if (rt * 16 < m) {
-#pragma unroll
-      for (int kq = 0; kq < 8; ++kq)
-        if (4 * kq < nbj) acc[s] = mfma(vtile(s, kq), Wf[4 * kq + lk][li], acc[s]);
+      for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
     }
+    lds_barrier();
+    // phase 3: C += V Wf, then back to the row-major-lanes layout and out
 #pragma unroll
-    for (int r = 0; r < 4; ++r) X[crow(lk, r)][li] = acc[s][r];
-    const int row = rt * 16 + li;
+    for (int s = 0; s < RT; ++s) {
+      const int rt = wave + NW * s;
+      if (rt * 16 < m) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double v = X[li][4 * r + lk];
-      if (cp[r] && row < m) cp[r][row] = v;
+        for (int kq = 0; kq < 8; ++kq)
+          if (4 * kq < nbj) acc[s] = mfma(vtile(s, kq), Wf[4 * kq + lk][li], acc[s]);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[crow(lk, r)][li] = acc[s][r];
+      const int row = rt * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = X[li][4 * r + lk];
+        if (cp[r] && row < m) cp[r][row] = v;
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cp[r] = cn[r];
     }
   }
 }
