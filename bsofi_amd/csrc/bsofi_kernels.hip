@@ -497,15 +497,6 @@ __device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, in
   return V[(size_t)col * ldv + row];
 }
 
-// Workgroup barrier that orders LDS only: waits for the wave's LDS
-// operations, not for its global loads, so that loads issued before it stay
-// in flight across it (__syncthreads also waits for every global access).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // The target slab lives in MFMA accumulator registers: NW waves (4 or 8);
 // wave w owns row tiles w, w+NW, ... (RT of them, m <= 16 NW RT).  Eight
 // waves halve the loads each lane has in flight at entry (a wave keeps at
@@ -522,9 +513,9 @@ __device__ __forceinline__ void lds_barrier() {
 // MS: a workgroup takes slabs it.sub .. it.sub + spw - 1 of its task, one
 // after the other: V (in registers) and T (in LDS) are loaded once, and the
 // next slab's C is loaded while the current one goes through phases 2 and 3
-// and is stored (the barriers order LDS only).  Without MS a workgroup takes
-// the one slab it.sub.  Every slab goes through the same arithmetic either
-// way.
+// and is stored (a workgroup barrier waits for LDS operations only, so the
+// loads stay in flight across it).  Without MS a workgroup takes the one slab
+// it.sub.  Every slab goes through the same arithmetic either way.
 template <int RT, int NW, bool MS = false>
 __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
                                                              const Item* __restrict__ items, int spw) {
@@ -652,7 +643,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
       columns(sub + 1, cn);
       load_c(cn);
     }
-    lds_barrier();
+    __syncthreads();
     // phase 2: Wf = -op(T) W
     for (int e = tid; e < 32 * 16; e += NT) {
       const int i = e >> 4, j = e & 15;
@@ -666,7 +657,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
       }
       W[i][j] = v;
     }
-    lds_barrier();
+    __syncthreads();
     // op(T) W on MFMA: waves 0 and 1 form rows 0-15 and 16-31 (K = 32).
     // Tl is zero outside the nbj x nbj triangle, so no masking is needed.
     if (wave < 2) {
@@ -680,7 +671,7 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
 #pragma unroll
       for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
     }
-    lds_barrier();
+    __syncthreads();
     // phase 3: C += V Wf, then back to the row-major-lanes layout and out
 #pragma unroll
     for (int s = 0; s < RT; ++s) {
