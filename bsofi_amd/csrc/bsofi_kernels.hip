@@ -1127,7 +1127,11 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
                                                                 const Item* __restrict__ items, int ldcar) {
   constexpr int NT = 64 * NW;
   constexpr int MT = 32;
+  constexpr int XLD = 36;  // column stride of Xs: the A-fragment reads hit 16 distinct bank pairs
   extern __shared__ double carry[];  // carry[kk * ldcar + (i & 3) * 8 + (i >> 2)] = row i, column kk
+  // X_{r,k} (the panel's 32 rows of block column k of A), staged by direct
+  // global -> LDS loads while block k + 1's carry part runs: Xs[kk * XLD + i]
+  __shared__ double Xs[256 * XLD];
   const Item it = items[blockIdx.x];
   const ScanTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1136,6 +1140,16 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   const int R0 = t.r0 + it.sub * MT;
   const gptr<double> A = glb(t.A);
   auto cidx = [&](int kk, int i) { return kk * ldcar + (i & 3) * 8 + (i >> 2); };
+  // One wave instruction moves column kk of X_{r,kx} (32 rows, 256 bytes):
+  // lane l loads bytes 4l .. 4l + 3, and the instruction writes them to
+  // Xs + kk * XLD + l / 2 doubles.  Rows past the matrix end read row N - 1
+  // (their products are masked).
+  const int srow = min(R0 + (lane >> 1), N - 1);
+  auto stage_col = [&](int kx, int kk) {
+    const char* src = reinterpret_cast<const char*>(t.A + ((size_t)kx * n + kk) * lda + srow) + 4 * (lane & 1);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(Xs + kk * XLD), 4, 0, 0);
+  };
 
   // rows of this lane's A values: R0 + 4 g + y
   int rblk[8], rin[8];
@@ -1162,6 +1176,12 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   const int rbmin = R0 / n;
   const bool diag = t.diag_only != 0;
   const int kstop = diag ? max(rbmin - 1, 0) : 0;
+  // the X part of block k runs when rbmin <= k: stage the first one now
+  if (rbmin <= p - 2) {
+    for (int kk = wave; kk < n; kk += NW) stage_col(p - 2, kk);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's columns are in LDS
+  }
+  __syncthreads();
   // Q_k^T columns of this lane's tiles.  When the 16-column tiles do not
   // divide evenly over the waves, a wave's last tiles lie beyond 2n: they
   // read the last column (in bounds) and their results are never stored.
@@ -1217,13 +1237,12 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
       step(1, masked_tag);
     };
     if (rbmin <= k) {
-      // [X_{r,k}] part: K rows 0..n-1 of Q_k^T
-      const gptr<const double> Xk = A + (size_t)k * n * lda;
+      // [X_{r,k}] part: K rows 0..n-1 of Q_k^T, X_{r,k} from Xs
       if (R0 + MT <= N && (R0 + MT - 1) / n < k) {  // every row below block row k's diagonal: no masks
         run_part(
             [&](int u, int k0) {
               const int kk = k0 + x;
-              const gptr<const double> xr = Xk + (size_t)kk * lda + R0 + y;
+              const double* xr = Xs + kk * XLD + y;
 #pragma unroll
               for (int g = 0; g < 8; ++g) af[u][g] = xr[4 * g];
 #pragma unroll
@@ -1234,10 +1253,11 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
         run_part(
             [&](int u, int k0) {
               const int kk = k0 + x;
+              const double* xr = Xs + kk * XLD + y;
               int mk = 0;
 #pragma unroll
               for (int g = 0; g < 8; ++g) {
-                af[u][g] = Xk[(size_t)kk * lda + min(R0 + 4 * g + y, N - 1)];
+                af[u][g] = xr[4 * g];
                 mk |= (rblk[g] <= k && !(rblk[g] == k && rin[g] > kk)) << g;
               }
 #pragma unroll
@@ -1262,6 +1282,14 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
           std::false_type{});
     }
     __syncthreads();
+    // When block k - 1 has an X part, X_{r,k-1} moves into Xs now (wave w:
+    // columns w, w + 4, ...), its loads in flight under the epilogue; a
+    // workgroup barrier waits for them.  (Mixed into the k-loop, a
+    // direct-to-LDS load among the loads of Q_k^T makes the compiler drain
+    // every load at each k-step.)
+    const bool stage = k - 1 >= kstop && rbmin <= k - 1;
+    if (stage)
+      for (int kk = wave; kk < n; kk += NW) stage_col(k - 1, kk);
     // epilogue: lane holds rows 4 g + x, column tile * 16 + l16
 #pragma unroll
     for (int b = 0; b < NTW; ++b) {
@@ -1280,6 +1308,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
         }
       }
     }
+    if (stage) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's columns of X_{r,k-1} are in LDS
     __syncthreads();
   }
   // block column 0 (the carry after k = 0), for diag_only only its diagonal
