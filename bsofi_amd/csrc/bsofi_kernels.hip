@@ -1159,7 +1159,6 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
     rblk[g] = row < N ? row / n : p;
     rin[g] = row - rblk[g] * n;
   }
-  const int ntiles = (n2 + 15) / 16;
 
   for (int e = tid; e < n * MT; e += NT) {
     const int i = e % MT, kk = e / MT;
@@ -1282,31 +1281,32 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
           std::false_type{});
     }
     __syncthreads();
-    // When block k - 1 has an X part, X_{r,k-1} moves into Xs now (wave w:
-    // columns w, w + 4, ...), its loads in flight under the epilogue; a
-    // workgroup barrier waits for them.  (Mixed into the k-loop, a
-    // direct-to-LDS load among the loads of Q_k^T makes the compiler drain
-    // every load at each k-step.)
+    // epilogue: lane holds rows 4 g + x, column tile * 16 + l16.  First the
+    // G block (columns n..2n-1 of the product), then X_{r,k-1} into Xs when
+    // block k - 1 has an X part (wave w: columns w, w + 4, ...), then the
+    // new carry, so that the stores' and the staging loads' latencies pass
+    // under the carry writes; the barrier waits for both.  (Mixed into the
+    // k-loop, a direct-to-LDS load among the loads of Q_k^T makes the
+    // compiler drain every load at each k-step.)
+#pragma unroll
+    for (int b = 0; b < NTW; ++b) {
+      const int col = (wave + NW * b) * 16 + l16;
+      if (col < n || col >= n2) continue;
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int row = R0 + 4 * g + x;
+        if (row < N && (!diag || row / n == k + 1)) A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[g][b];
+      }
+    }
     const bool stage = k - 1 >= kstop && rbmin <= k - 1;
     if (stage)
       for (int kk = wave; kk < n; kk += NW) stage_col(k - 1, kk);
-    // epilogue: lane holds rows 4 g + x, column tile * 16 + l16
 #pragma unroll
     for (int b = 0; b < NTW; ++b) {
-      const int tile = wave + NW * b;
-      if (tile >= ntiles) continue;
-      const int col = tile * 16 + l16;
-      if (col >= n2) continue;
+      const int col = (wave + NW * b) * 16 + l16;
+      if (col >= n) continue;
 #pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        const int i = 4 * g + x;
-        if (col < n) {
-          carry[cidx(col, i)] = acc[g][b];
-        } else {
-          const int row = R0 + i;
-          if (row < N && (!diag || row / n == k + 1)) A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[g][b];
-        }
-      }
+      for (int g = 0; g < 8; ++g) carry[cidx(col, 4 * g + x)] = acc[g][b];
     }
     if (stage) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's columns of X_{r,k-1} are in LDS
     __syncthreads();
