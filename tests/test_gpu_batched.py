@@ -135,6 +135,55 @@ def test_one_lane_same_bits(B):
             assert np.array_equal(out[key][k], out[1, False][k]), f"case {k} differs with lanes/graph {key}"
 
 
+@pytest.mark.parametrize("lx,p,diag", [(12, 5, False), (8, 6, False), (12, 4, True)])
+def test_single_matrix_plan_graph_and_lanes_same_bits(B, eps, lx, p, diag):
+    """A single-matrix plan (which runs in one lane whatever the context's
+    setting): the same bits with the context at one lane or two, eagerly and
+    over graph replays with eager work between them, and the inverse (or its
+    diagonal blocks) checks out against the oracle."""
+    import torch
+
+    A0, m = dqmc.dqmc_matrix(lx, p, instance=7)
+    n = m["n"]
+    phases = B.PHASE_ALL | (B.PHASE_DIAG_ONLY if diag else 0)
+    out = {}
+    for lanes in (1, 2):
+        for graph in (False, True):
+            ctx = B.Context(0)
+            ctx.set_lanes(lanes)
+            d = to_dev(A0)
+            t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            plan = B.Plan(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)], phases=phases)
+            plan.use_graph(graph)
+            reps = 3 if graph else 1
+            for r in range(reps):
+                print(f"n={n} p={p} diag={diag} lanes={lanes} graph={graph} replay={r}", flush=True)
+                d.copy_(to_dev(A0))
+                t.zero_()
+                torch.cuda.synchronize()
+                plan.execute()
+                ctx.synchronize()
+                assert plan.info() == [0]
+                got = from_dev(d, n * p).copy()
+                if (lanes, graph) in out:
+                    assert np.array_equal(got, out[lanes, graph])
+                out[lanes, graph] = got
+            plan.close()
+            ctx.close()
+    for key in out:
+        assert np.array_equal(out[key], out[1, False]), f"lanes/graph {key} differ from one lane, eager"
+    G = out[2, True]
+    if diag:
+        full = A0.copy(order="F")
+        oracle.Oracle("auto", threads=1).invert(n, p, full)
+        for l in range(p):
+            blk = np.s_[l * n:(l + 1) * n, l * n:(l + 1) * n]
+            assert rel(G[blk], full[blk]) <= 1e-10
+    else:
+        assert oracle.residual(A0, G) <= 100 * n * p * eps
+
+
 def test_graph_replays_match_eager(B):
     import torch
 
