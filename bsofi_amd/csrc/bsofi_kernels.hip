@@ -487,9 +487,10 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
 // ---------------------------------------------------------------------------
 // Block reflector application (K2)
 // ---------------------------------------------------------------------------
-// One workgroup = one 16-column slab of the target columns, all m rows.
-//   phase 1: W = V^T C        (nbj x 16, K = m split over the 4 waves)
-//   phase 2: W <- -op(T) W    (LDS, VALU)
+// One workgroup = one 16-column slab of the target columns, all m rows (in
+// the multi-slab form, spw consecutive slabs one after the other).
+//   phase 1: W = V^T C        (nbj x 16, K = m split over the 4 or 8 waves)
+//   phase 2: W <- -op(T) W    (partials summed in LDS, the product on MFMA)
 //   phase 3: C <- C + V W     (m x 16, K = nbj)
 __device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, int nbj, int row, int col) {
   if (row >= m || col >= nbj || row < col) return 0.0;
@@ -1195,15 +1196,15 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
 #pragma unroll
       for (int b = 0; b < NTW; ++b) acc[g][b] = 0.0;
 
-    // Two register buffers: the loads of k-step s + 1 are in flight while
-    // the MFMAs of step s run (and the other wave of the SIMD covers what
-    // is left of their latency).  With 8 | n (and the clamped columns)
-    // every index of the carry part and of Q_k^T is in range, so only the X part of a
-    // workgroup whose rows reach the diagonal block (or the matrix end)
-    // needs masks: there loads are unconditional, from indexes clamped into
-    // the matrix, and the masks (bit g: row group g) are applied just before
-    // the MFMAs.  (A select right after a load, or a predicated load, makes
-    // the compiler wait for the loads early.)
+    // Two register buffers: the loads of k-step s + 1 (Q_k^T from L2, the
+    // A fragments from LDS) are in flight while the 8 NTW MFMAs of step s
+    // run; with one wave per SIMD nothing else covers their latency, which
+    // is why X_{r,k} comes from LDS and not from HBM.  With 8 | n (and the
+    // clamped columns) every index of the carry part and of Q_k^T is in
+    // range, so only the X part of a workgroup whose rows reach the diagonal
+    // block (or the matrix end) needs masks, applied to the A values just
+    // before the MFMAs (bit g: row group g).  (A select right after a load,
+    // or a predicated load, makes the compiler wait for the loads early.)
     double af[2][8], bf[2][NTW];
     int msk[2];
     auto step = [&](int u, auto masked_tag) {
