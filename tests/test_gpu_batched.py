@@ -106,6 +106,32 @@ def test_many_slab_reflector_launch_same_bits(B, eps):
     ctx.close()
 
 
+def test_large_same_size_batch_same_bits(B, eps):
+    """64 matrices of n = 144, p = 4 in one plan (two lanes, BSTRI steps of a
+    few hundred GEMM tasks, multi-slab reflector launches): three of them
+    get the bits they get alone, and check out against the oracle."""
+    import torch
+
+    ctx = B.Context(0)
+    n, p = 144, 4
+    cases = [dqmc.dqmc_matrix(12, p, instance=100 + k)[0] for k in range(64)]
+    devs = [to_dev(A0) for A0 in cases]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for _ in cases]
+    torch.cuda.synchronize()
+    info = B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t in zip(devs, taus)])
+    assert info == [0] * len(cases)
+    for k in (0, 37, 63):
+        print(f"large-batch case {k}: n={n} p={p}", flush=True)
+        d = to_dev(cases[k])
+        t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+        alone = from_dev(d, n * p)
+        assert np.array_equal(alone, from_dev(devs[k], n * p)), f"case {k} differs between alone and batch"
+        assert oracle.residual(cases[k], alone) <= 100 * n * p * eps
+    ctx.close()
+
+
 def test_one_lane_same_bits(B):
     """A batch plan in one lane (every launch on the context's stream) and in
     two (the default) gives the same bits, eagerly and as a graph."""
