@@ -649,14 +649,29 @@ struct Planner {
       std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.cost > b.cost; });
       // XCD-aware order.  Workgroup i runs on XCD i mod 8, each XCD with its
       // own L2.  Row panels of one block row (one cost) step through k in
-      // lockstep and read the same Q_k, so whole block rows are dealt to
-      // XCDs round-robin (balanced work, shared L2 lines), heaviest first.
+      // lockstep and read the same Q_k, so whole block rows go to one XCD.
+      // An XCD takes consecutive block rows, as many as it runs at once
+      // (their X parts end within a few k of each other, so the panels drift
+      // little apart and keep sharing Q_k in L2), and the rounds snake over
+      // the XCDs (0..7, then 7..0), so the XCD with the heaviest rows of one
+      // round gets the lightest of the next.
       constexpr int kXcd = 8;
       std::vector<std::vector<Item>> q(kXcd);
-      int g = -1;
+      const size_t per_round = std::max<size_t>(1, cus / kXcd);
+      int xcd = 0, dir = 1;
+      size_t filled = 0;
       for (size_t i = 0; i < order.size(); ++i) {
-        if (i == 0 || order[i].cost != order[i - 1].cost || order[i].task != order[i - 1].task) ++g;
-        q[g % kXcd].push_back({order[i].task, order[i].sub});
+        const bool new_row = i == 0 || order[i].cost != order[i - 1].cost || order[i].task != order[i - 1].task;
+        if (new_row && filled >= per_round) {
+          filled = 0;
+          xcd += dir;
+          if (xcd == kXcd || xcd < 0) {
+            dir = -dir;
+            xcd += dir;
+          }
+        }
+        q[xcd].push_back({order[i].task, order[i].sub});
+        ++filled;
       }
       std::vector<size_t> head(kXcd, 0);
       for (size_t i = 0; i < order.size();) {
