@@ -96,7 +96,9 @@ static constexpr int kScan4 = 9;
 static int scan_rows(int mti) { return mti == kScan4 ? 32 : 16 * mti; }
 
 static bool scan_variant_ok(int mti, int ntw) {
-  if (mti == kScan4) return ntw >= 1 && ntw <= 8;  // with 8 | n (scan_ok)
+  // with 8 | n (scan_ok); ntw <= 8 means n <= 256, the column count of the
+  // kernel's LDS staging buffer for X_{r,k}
+  if (mti == kScan4) return ntw >= 1 && ntw <= 8;
   if (mti == 1) return ntw >= 1 && ntw <= 16;
   if (mti == 2) return ntw >= 1 && ntw <= 8;
   if (mti == 4) return ntw >= 1 && ntw <= 4;

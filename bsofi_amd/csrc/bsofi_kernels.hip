@@ -1131,8 +1131,10 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   constexpr int XLD = 36;  // column stride of Xs: the A-fragment reads hit 16 distinct bank pairs
   extern __shared__ double carry[];  // carry[kk * ldcar + (i & 3) * 8 + (i >> 2)] = row i, column kk
   // X_{r,k} (the panel's 32 rows of block column k of A), staged by direct
-  // global -> LDS loads while block k + 1's carry part runs: Xs[kk * XLD + i]
+  // global -> LDS loads during block k + 1's epilogue: Xs[kk * XLD + i].
+  // The planner takes this form for n <= 256 only (NTW <= 8).
   __shared__ double Xs[256 * XLD];
+  static_assert(NTW <= 8, "n <= 256: X_{r,k} must fit Xs");
   const Item it = items[blockIdx.x];
   const ScanTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
