@@ -117,10 +117,13 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   g.nb = (tune && tune->nb) ? tune->nb : nb_for(n);
   g.S = (2 * n + g.nb - 1) / g.nb;
   const int m = 2 * n;
-  const int rpg = m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
-  // panel_factor_kernel<RPG, CPW, NW>: 8 waves of nb / 8 columns for the
-  // 32-wide sub-panel of 2n in (256, 512], else 4 waves of nb / 4 columns
-  const int nw = (g.nb == 32 && m > 256 && m <= 512) ? 8 : 4;
+  // panel_factor_kernel<RPG, CPW, NW>: one instance, 8 waves of 4 columns
+  // over 512 rows, for every 32-wide sub-panel of 2n <= 512, so that a batch
+  // round of such matrices takes one panel launch; else 4 waves of nb / 4
+  // columns over the next power-of-two row count
+  const bool p8 = g.nb == 32 && m <= 512;
+  const int rpg = p8 ? 8 : m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
+  const int nw = p8 ? 8 : 4;
   g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
   // default scan form: the 4x4x4 scan from n = 128 (one workgroup per CU;
   // below that its k-loops are short and batches of small matrices need the
