@@ -761,7 +761,6 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     hipLaunchKernelGGL((panel_factor_kernel<R, C, W>), dim3(L.grid), dim3(64 * W), 0, s, t, it); \
     break;
       switch (L.variant) {
-        PANEL_CASE(2, 8, 4) PANEL_CASE(4, 8, 4) PANEL_CASE(8, 8, 4)
         PANEL_CASE(2, 4, 4) PANEL_CASE(4, 4, 4) PANEL_CASE(8, 4, 4) PANEL_CASE(16, 4, 4)
         PANEL_CASE(2, 2, 4) PANEL_CASE(4, 2, 4) PANEL_CASE(8, 2, 4) PANEL_CASE(16, 2, 4) PANEL_CASE(32, 2, 4)
         PANEL_CASE(8, 4, 8)
