@@ -54,7 +54,10 @@ PHASE_CASES = [("dqmc", 2, 2), ("dqmc", 2, 3), ("dqmc", 2, 4), ("dqmc", 3, 5), (
                # 2n above 512: the m <= 1024 variants of the panel and reflector kernels
                ("rand_eye", 300, 3), ("dqmc", 20, 2),
                # 8 | n in [128, 256]: the 4x4x4 scan by default
-               ("rand_eye", 128, 3), ("rand_full", 160, 2)]
+               ("rand_eye", 128, 3), ("rand_full", 160, 2),
+               # n = 144, p = 16: BSOI's Q_k formation steps hold more slabs than the GPU has CUs, so a
+               # single-matrix plan takes the multi-slab reflector form
+               ("dqmc", 12, 16)]
 
 
 @pytest.mark.parametrize("kind,a,p", PHASE_CASES)
