@@ -432,8 +432,10 @@ struct Planner {
 
   // -- BSTRI: src/bsofi_cpu.c:146-177, restructured by block rows ---------
   // GEMM tasks of one launch step: 128 x 128 tiles where a product still
-  // yields >= 192 of them (enough to fill the GPU), 64 x 64 tiles otherwise.
-  static bool gemm_big(int M, int N) { return (long long)(M / 128) * (N / 128) >= 192; }
+  // yields a full round of them on the 256 CUs, 64 x 64 tiles otherwise (at
+  // 192 the n = 256 row sweep's largest products took 128-wide tiles and the
+  // plain inversion was 0.8 % slower).
+  static bool gemm_big(int M, int N) { return (long long)(M / 128) * (N / 128) >= 256; }
 
   struct GemmBatch {
     std::vector<GemmTask> t64, t128;
