@@ -397,8 +397,8 @@ def main():
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs are out of scope "
-              "(see the sharding design in DESIGN.md).", file=sys.stderr)
+        print("bench.py: one GPU only; multi-GPU runs are out of scope here (the path would shard by "
+              "row panels of G, each GPU forming every Q_k and scanning its own rows).", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy > 0:

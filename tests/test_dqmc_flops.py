@@ -56,7 +56,7 @@ def test_assembled_pattern():
 
 
 def test_flops_match_published_figures():
-    # BASELINE.md section 2 (from include/bsofi_flops.h:19-30)
+    # FLOPS_BSOFI of include/bsofi_flops.h:19-30 evaluated by hand at these sizes
     assert abs(flops.bsofi(64, 16) - 5.228e8) / 5.228e8 < 1e-3
     assert abs(flops.bsofi(256, 64) - 4.958e11) / 4.958e11 < 1e-3
     assert abs(flops.bsofi(64, 32) - 1.996e9) / 1.996e9 < 1e-3
