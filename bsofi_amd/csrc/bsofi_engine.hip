@@ -653,7 +653,21 @@ struct Planner {
           order.push_back({cost, id, s});
         }
       }
-      std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.cost > b.cost; });
+      // Heaviest row panels first.  When every matrix of the group has the
+      // same n and p, that order only interleaves identical matrices, so it
+      // goes matrix by matrix instead (each one's block rows heaviest first):
+      // the panels an XCD runs at once then belong to one matrix and share
+      // its Q_k in L2 (interleaved, 512 matrices of n = 64 re-read each Q_k
+      // from HBM about 30 times; C3 +5 %).  Uneven groups keep the cost order,
+      // whose longest-first schedule matters more there (C4 -1.4 % otherwise).
+      bool uniform = true;
+      for (auto& t : st) uniform = uniform && t.n == st[0].n && t.p == st[0].p;
+      if (uniform)
+        std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) {
+          return a.task != b.task ? a.task < b.task : a.cost > b.cost;
+        });
+      else
+        std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.cost > b.cost; });
       // XCD-aware order.  Workgroup i runs on XCD i mod 8, each XCD with its
       // own L2.  Row panels of one block row (one cost) step through k in
       // lockstep and read the same Q_k, so whole block rows go to one XCD.
