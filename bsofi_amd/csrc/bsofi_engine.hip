@@ -356,6 +356,22 @@ struct Planner {
     std::vector<Item> items;
     for (size_t k = 0; k < R.t.size(); ++k)
       for (int s = 0; s < R.slabs[k]; s += spw) items.push_back({(int)k, s});
+    // Workgroup i runs on XCD i mod 8: deal the tasks to XCDs round-robin and
+    // interleave, so that all workgroups of a task run on one XCD and read
+    // its V through one L2 (task by task, a task's slabs spread over every
+    // XCD and each fetched V from HBM).
+    if (R.t.size() > 1) {
+      constexpr int kXcd = 8;
+      std::vector<std::vector<Item>> q(kXcd);
+      for (const Item& it : items) q[it.task % kXcd].push_back(it);
+      std::vector<Item> dealt;
+      dealt.reserve(items.size());
+      std::vector<size_t> head(kXcd, 0);
+      while (dealt.size() < items.size())
+        for (int x = 0; x < kXcd; ++x)
+          if (head[x] < q[x].size()) dealt.push_back(q[x][head[x]++]);
+      items.swap(dealt);
+    }
     const size_t before = out.size();
     emit(F_REFL, variant, R.t, items);
     if (out.size() > before) out.back().spw = std::max(spw, 1);
