@@ -1,6 +1,6 @@
 """GPU parity: the HIP path against the oracle on the same seeded inputs.
 
-Tolerances (FP64 throughout, from the north-star bar):
+Tolerances (FP64 throughout):
 * every inverse: ||A G - I||_F / (||A||_F ||G||_F) <= 100 N eps;
 * beta = p * dtau <= 4: ||G - G_ref||_F / ||G_ref||_F <= 1e-10;
 * intermediates (A = V + R after BSOF, tau, A = V + R^{-1} after BSTRI) are
