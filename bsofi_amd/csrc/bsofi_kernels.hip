@@ -373,7 +373,10 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
       x[0] = (lane > c) ? a[0][jc] : 0.0;
 #pragma unroll
       for (int q = 1; q < RPG; ++q) x[q] = a[q][jc];
-      double n0 = x[0] * x[0], n1 = 0.0;
+      // Both chains start with an fma, never a product the compiler could
+      // contract into the final add: the panel instances (RPG) then round
+      // identically, the extra rows of a larger RPG adding exact zeros.
+      double n0 = fma(x[0], x[0], 0.0), n1 = 0.0;
 #pragma unroll
       for (int q = 1; q < RPG; ++q) {
         if (q & 1) n1 = fma(x[q], x[q], n1);
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
           double P[CPW];
 #pragma unroll
           for (int jj = 0; jj < CPW; ++jj) {
-            double s0 = v[0] * a[0][jj], s1 = 0.0;
+            double s0 = fma(v[0], a[0][jj], 0.0), s1 = 0.0;  // fma start, as for the norm
 #pragma unroll
             for (int q = 1; q < RPG; ++q) {
               if (q & 1) s1 = fma(v[q], a[q][jj], s1);
