@@ -117,10 +117,11 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   g.nb = (tune && tune->nb) ? tune->nb : nb_for(n);
   g.S = (2 * n + g.nb - 1) / g.nb;
   const int m = 2 * n;
-  // panel_factor_kernel<RPG, CPW, NW>: one instance, 8 waves of 4 columns
-  // over 512 rows, for every 32-wide sub-panel of 2n <= 512, so that a batch
-  // round of such matrices takes one panel launch; else 4 waves of nb / 4
-  // columns over the next power-of-two row count
+  // panel_factor_kernel<RPG, CPW, NW>: 8 waves of 4 columns for every
+  // 32-wide sub-panel of 2n <= 512, so that a batch round of such matrices
+  // takes one panel launch (over 512 rows here; the round's largest m picks
+  // 128 or 256 where it can); else 4 waves of nb / 4 columns over the next
+  // power-of-two row count
   const bool p8 = g.nb == 32 && m <= 512;
   const int rpg = p8 ? 8 : m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
   const int nw = p8 ? 8 : 4;
