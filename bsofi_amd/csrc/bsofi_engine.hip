@@ -1172,16 +1172,16 @@ static void run_all(bsofi_plan_s* P) {
   for (const Launch* Lp : order) {
     const Launch& L = *Lp;
     hipStream_t s = st[L.lane & 1];
-    const bool timed = (L.family == P->timed_family);
+    bool timed = (L.family == P->timed_family);
     if (timed) {
-      if ((size_t)P->ev_used + 2 > P->ev.size()) {
-        for (int k = 0; k < 2; ++k) {
-          hipEvent_t e;
-          (void)hipEventCreate(&e);
-          P->ev.push_back(e);
-        }
+      // a launch whose events cannot be created runs untimed
+      while ((size_t)P->ev_used + 2 > P->ev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) break;
+        P->ev.push_back(e);
       }
-      (void)hipEventRecord(P->ev[P->ev_used], s);
+      timed = (size_t)P->ev_used + 2 <= P->ev.size();
+      if (timed) (void)hipEventRecord(P->ev[P->ev_used], s);
     }
     run_launch(L, P->d_blob, s);
     if (timed) {
