@@ -501,8 +501,10 @@ __device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, in
   return V[(size_t)col * ldv + row];
 }
 
-// The target slab lives in MFMA accumulator registers: NW waves (4 or 8);
-// wave w owns row tiles w, w+NW, ... (RT of them, m <= 16 NW RT).  Eight
+// The target slab lives in MFMA accumulator registers: NW waves (4 or 8;
+// the engine launches the 8-wave form, whose sums the 4-wave form repeats
+// in the same order); wave w owns row tiles w, w+NW, ... (RT of them,
+// m <= 16 NW RT).  Eight
 // waves halve the loads each lane has in flight at entry (a wave keeps at
 // most 63 outstanding) and put two waves per SIMD.  The C layout of a
 // 16 x 16 accumulator tile is exactly four consecutive B fragments of a
