@@ -480,8 +480,23 @@ struct Planner {
     }
   };
 
+  // A launch of fewer 64 x 64 tiles than CUs takes 32 x 32 tiles instead:
+  // four times the workgroups, each a quarter of the MFMA chain.  The k order
+  // of every element is the same for every tile size, so are the bits.
   void emit_gemm(const GemmBatch& b) {
-    emit(F_GEMM, 0, b.t64, b.i64);
+    if (!b.i64.empty() && b.i64.size() < cus) {
+      std::vector<GemmTask> t32 = b.t64;
+      std::vector<Item> i32;
+      for (size_t id = 0; id < t32.size(); ++id) {
+        GemmTask& t = t32[id];
+        t.tiles_n = (t.N + 31) / 32;
+        const int tm = (t.M + 31) / 32;
+        for (int s = 0; s < tm * t.tiles_n; ++s) i32.push_back({(int)id, s});
+      }
+      emit(F_GEMM, 2, t32, i32);
+    } else {
+      emit(F_GEMM, 0, b.t64, b.i64);
+    }
     emit(F_GEMM, 1, b.t128, b.i128);
   }
 
@@ -826,7 +841,10 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_GEMM:
-      if (L.variant == 1)
+      if (L.variant == 2)
+        hipLaunchKernelGGL(gemm_kernel<1>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
+                           (const Item*)(base + L.item_off));
+      else if (L.variant == 1)
         hipLaunchKernelGGL(gemm_kernel<4>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
                            (const Item*)(base + L.item_off));
       else

@@ -798,7 +798,7 @@ __global__ void fill_int_kernel(int* __restrict__ p, int count, int value) {
 
 // ---------------------------------------------------------------------------
 // Tile GEMM (K4): C = alpha op(A) op(B) + beta C (+ D), 32 WT x 32 WT output
-// tile per workgroup (WT = 2: 64 x 64, WT = 4: 128 x 128), each wave a
+// tile per workgroup (WT = 1: 32 x 32, 2: 64 x 64, 4: 128 x 128), each wave a
 // 16 WT x 16 WT quarter (WT x WT MFMA tiles).  K runs in chunks of 16
 // through two LDS buffers: the global loads of chunk c + 1 are in flight
 // while chunk c is multiplied, one barrier per chunk.  Global reads are
