@@ -331,8 +331,8 @@ struct Planner {
   // Reflector tasks of one launch step.  The kernel instance follows from
   // the largest m of the step; every instance sums a task's partial
   // products in the same order, so a task's bits do not depend on the other
-  // matrices of the launch.  In the 8-wave, 4-tile instance a workgroup
-  // takes spw consecutive 16-column slabs of one task (variant 204): 1 while
+  // matrices of the launch.  In the 8-wave instances up to 4 tiles a workgroup
+  // takes spw consecutive 16-column slabs of one task (variant 200 + RT): 1 while
   // the slabs fit one round of workgroups on the GPU (each workgroup is
   // latency-bound, so more slabs per workgroup would only lengthen the
   // launch), else as many as it takes to fit one round, at most 8, so that V
@@ -350,8 +350,9 @@ struct Planner {
   };
   void emit_refl(const ReflRound& R) {
     int variant = refl_variant(R.t);
-    const int spw = variant == 104 ? (int)std::min<size_t>(8, (R.total + cus - 1) / cus) : 1;
-    if (spw > 1) variant = 204;
+    const size_t round = cus * (variant == 101 ? 2 : 1);  // multi-slab workgroups per CU (VGPRs, LDS)
+    const int spw = variant <= 104 ? (int)std::min<size_t>(8, (R.total + round - 1) / round) : 1;
+    if (spw > 1) variant += 100;
     std::vector<Item> items;
     for (size_t k = 0; k < R.t.size(); ++k)
       for (int s = 0; s < R.slabs[k]; s += spw) items.push_back({(int)k, s});
@@ -833,6 +834,12 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
         case 101: hipLaunchKernelGGL((refl_apply_kernel<1, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
         case 102: hipLaunchKernelGGL((refl_apply_kernel<2, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
         case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
+        case 201:
+          hipLaunchKernelGGL((refl_apply_kernel<1, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
+          break;
+        case 202:
+          hipLaunchKernelGGL((refl_apply_kernel<2, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
+          break;
         case 204:
           hipLaunchKernelGGL((refl_apply_kernel<4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
           break;
