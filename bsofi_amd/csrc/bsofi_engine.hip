@@ -434,15 +434,15 @@ struct Planner {
     }
   }
 
-  // The 8-wave panel instance holds 64 * RPG rows per column; a round whose
-  // sub-panels all have m <= 128 (m <= 256) takes RPG = 2 (4), a shorter
-  // chain per column.  Rows beyond m enter only as zeros (fma(0, 0, s) = s)
-  // and are never stored, so the factors do not depend on the instance.
+  // The 8-wave panel instance holds 64 * RPG rows per column; a round takes
+  // the smallest RPG (2 .. 8) that covers its largest m, a shorter chain per
+  // column.  Rows beyond m enter only as zeros (fma(0, 0, s) = s) and are
+  // never stored, so the factors do not depend on the instance.
   static int panel_instance(int variant, const std::vector<PanelTask>& t) {
     if (variant != 80804) return variant;
     int mm = 0;
     for (const PanelTask& q : t) mm = std::max(mm, q.m);
-    return mm <= 128 ? 80204 : mm <= 256 ? 80404 : variant;
+    return 80004 + 100 * std::max(2, (mm + 63) / 64);
   }
 
   // sign(det A) and log|det A| from R's diagonal and tau, before BSTRI
@@ -821,7 +821,8 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       switch (L.variant) {
         PANEL_CASE(2, 4, 4) PANEL_CASE(4, 4, 4) PANEL_CASE(8, 4, 4) PANEL_CASE(16, 4, 4)
         PANEL_CASE(2, 2, 4) PANEL_CASE(4, 2, 4) PANEL_CASE(8, 2, 4) PANEL_CASE(16, 2, 4) PANEL_CASE(32, 2, 4)
-        PANEL_CASE(2, 4, 8) PANEL_CASE(4, 4, 8) PANEL_CASE(8, 4, 8)
+        PANEL_CASE(2, 4, 8) PANEL_CASE(3, 4, 8) PANEL_CASE(4, 4, 8) PANEL_CASE(5, 4, 8) PANEL_CASE(6, 4, 8)
+        PANEL_CASE(7, 4, 8) PANEL_CASE(8, 4, 8)
         default: break;
       }
 #undef PANEL_CASE
