@@ -317,15 +317,17 @@ struct Planner {
     push(L);
   }
 
-  // refl_apply_kernel instance for the largest m: 8 waves with RT = 1, 2, 4
-  // or 8 row tiles each (variant = 100 + RT, m <= 128 RT).  Below m = 256 the
-  // 8-wave instances replaced 4-wave ones with twice the tiles per wave: the
-  // same sums in the same order, half the MFMA chain per wave (the uneven
-  // matrices one at a time +1.9 %, batches unchanged).
+  // refl_apply_kernel instance for the largest m: 8 waves with the fewest
+  // row tiles each that cover it, RT = 1 .. 4, or 8 beyond m = 512 (variant =
+  // 100 + RT, m <= 128 RT).  Every RT sums a task's tiles on the same chains
+  // in the same order.  Below m = 256 the 8-wave instances replaced 4-wave
+  // ones with twice the tiles per wave, half the MFMA chain per wave (the
+  // uneven matrices one at a time +1.9 %, batches unchanged); RT = 3 for
+  // m <= 384 gave C2 +0.6 %.
   static int refl_variant(const std::vector<ReflTask>& rt) {
     int mm = 0;
     for (auto& q : rt) mm = std::max(mm, q.m);
-    return mm <= 128 ? 101 : mm <= 256 ? 102 : mm <= 512 ? 104 : 108;
+    return mm <= 512 ? 100 + std::max(1, (mm + 127) / 128) : 108;
   }
 
   // Reflector tasks of one launch step.  The kernel instance follows from
@@ -834,12 +836,16 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       switch (L.variant) {
         case 101: hipLaunchKernelGGL((refl_apply_kernel<1, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
         case 102: hipLaunchKernelGGL((refl_apply_kernel<2, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
+        case 103: hipLaunchKernelGGL((refl_apply_kernel<3, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
         case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
         case 201:
           hipLaunchKernelGGL((refl_apply_kernel<1, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
           break;
         case 202:
           hipLaunchKernelGGL((refl_apply_kernel<2, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
+          break;
+        case 203:
+          hipLaunchKernelGGL((refl_apply_kernel<3, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
           break;
         case 204:
           hipLaunchKernelGGL((refl_apply_kernel<4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
