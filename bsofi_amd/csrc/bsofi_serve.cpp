@@ -78,7 +78,9 @@ struct Slot {
   bsofi_plan_t plan = nullptr;
   hipEvent_t done = nullptr;
   std::vector<long long> reqs;
-  size_t ws_bytes = 0;
+  size_t ws_bytes = 0;     // work bytes of the batch, counted in `used` while it runs
+  void* plan_ws = nullptr;  // the plan's workspace, from the server's pool
+  size_t plan_ws_bytes = 0;
   bool busy = false;
 };
 
@@ -185,6 +187,7 @@ int bsofi_server_destroy(bsofi_server_t s) {
   for (auto& sl : s->slots) {
     if (sl.ctx) bsofi_ctx_synchronize(sl.ctx);
     if (sl.plan) bsofi_plan_destroy(sl.plan);
+    if (sl.plan_ws) (void)hipFree(sl.plan_ws);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.ctx) bsofi_ctx_destroy(sl.ctx);
   }
@@ -244,8 +247,12 @@ int bsofi_server_submit(bsofi_server_t s, int chain, const int* flips, int nflip
   if (chain < 0 || chain >= (int)s->chains.size()) return -2;
   Chain& c = s->chains[chain];
   const int total = c.n * c.p;
+  if (nflips < 0 || (nflips > 0 && !flips)) return -4;
   for (int k = 0; k < nflips; ++k)
     if (flips[k] < 0 || flips[k] >= total) return -3;
+  // a request larger than the whole budget could never be admitted and
+  // would block the FIFO head for good
+  if (result_bytes(c.n, c.p) + work_bytes(c.n, c.p) > s->budget) return BSOFI_ERR_ALLOC;
   for (int k = 0; k < nflips; ++k) c.fields[flips[k]] = (signed char)(-c.fields[flips[k]]);
   auto r = std::make_unique<Request>();
   r->id = s->next_id++;
@@ -260,6 +267,20 @@ int bsofi_server_submit(bsofi_server_t s, int chain, const int* flips, int nflip
   s->queue.push_back(r->id);
   s->reqs[r->id] = std::move(r);
   return 0;
+}
+
+// Returns a slot's device buffers to the pool and forgets its batch (after
+// it retired, or after a launch that failed part way).
+static void slot_clear(bsofi_server_t s, Slot& sl) {
+  if (sl.plan) bsofi_plan_destroy(sl.plan);
+  sl.plan = nullptr;
+  pool_put(s, sl.plan_ws, sl.plan_ws_bytes);
+  sl.plan_ws = nullptr;
+  sl.plan_ws_bytes = 0;
+  s->used -= sl.ws_bytes;
+  sl.ws_bytes = 0;
+  sl.reqs.clear();
+  sl.busy = false;
 }
 
 static int launch_slot(bsofi_server_t s, Slot& sl, std::vector<Request*>& batch) {
@@ -278,9 +299,18 @@ static int launch_slot(bsofi_server_t s, Slot& sl, std::vector<Request*>& batch)
     mats[i].p = r.p;
     mats[i].lda = (int)N;
   }
+  // The plan's workspace comes from the pool too, so that it is counted in
+  // the budget (the requests' work bytes cover it: a batch's workspace is at
+  // most the sum of its matrices' single-matrix workspaces).
+  sl.plan_ws_bytes = bsofi_plan_workspace_bytes((int)mats.size(), mats.data(), BSOFI_PHASE_ALL);
+  sl.plan_ws = pool_get(s, sl.plan_ws_bytes);
+  if (!sl.plan_ws) {
+    sl.plan_ws_bytes = 0;
+    return BSOFI_ERR_ALLOC;
+  }
   // The plan first (its creation waits for the slot's stream, which is idle
   // now), then the inputs are built and the plan runs behind them.
-  int rc = bsofi_plan_create(sl.ctx, (int)mats.size(), mats.data(), BSOFI_PHASE_ALL, nullptr, &sl.plan);
+  int rc = bsofi_plan_create(sl.ctx, (int)mats.size(), mats.data(), BSOFI_PHASE_ALL, sl.plan_ws, &sl.plan);
   if (rc) return rc;
   for (size_t i = 0; i < batch.size(); ++i) {
     Request& r = *batch[i];
@@ -295,6 +325,29 @@ static int launch_slot(bsofi_server_t s, Slot& sl, std::vector<Request*>& batch)
   if (hipEventRecord(sl.done, st) != hipSuccess) return BSOFI_ERR_NO_DEVICE;
   sl.busy = true;
   return 0;
+}
+
+// A batch whose launch failed: its requests end with info = rc (the caller
+// sees them in poll() and releases them), their buffers go back to the pool
+// once the slot's stream has drained, and the budget they held is returned.
+static void fail_batch(bsofi_server_t s, Slot& sl, std::vector<Request*>& batch, int rc) {
+  (void)bsofi_ctx_synchronize(sl.ctx);
+  const double t = now_s();
+  for (Request* r : batch) {
+    const size_t N = (size_t)r->n * r->p;
+    pool_put(s, r->dA, N * N * 8);
+    pool_put(s, r->dtau, N * 8);
+    pool_put(s, r->dfields, r->fields.size());
+    r->dA = nullptr;
+    r->dtau = nullptr;
+    r->dfields = nullptr;
+    s->used -= r->result_bytes;
+    r->result_bytes = 0;
+    r->state = DONE;
+    r->info = rc;
+    r->t_done = t;
+  }
+  slot_clear(s, sl);
 }
 
 int bsofi_server_step(bsofi_server_t s, int* inflight) {
@@ -319,12 +372,7 @@ int bsofi_server_step(bsofi_server_t s, int* inflight) {
       r.dfields = nullptr;
       s->completed++;
     }
-    bsofi_plan_destroy(sl.plan);
-    sl.plan = nullptr;
-    s->used -= sl.ws_bytes;  // the batch's work bytes; results stay until release
-    sl.ws_bytes = 0;
-    sl.reqs.clear();
-    sl.busy = false;
+    slot_clear(s, sl);  // the batch's work bytes; results stay until release
   }
   // 2./3. admit into free slots within the budget, FIFO
   for (size_t si = 0; si < s->slots.size() && !s->queue.empty(); ++si) {
@@ -353,7 +401,7 @@ int bsofi_server_step(bsofi_server_t s, int* inflight) {
     sl.ws_bytes = 0;
     for (Request* r : batch) sl.ws_bytes += r->work_bytes;
     int rc = launch_slot(s, sl, batch);
-    if (rc) return rc;
+    if (rc) fail_batch(s, sl, batch, rc);
   }
   if (inflight) {
     int k = 0;

@@ -110,6 +110,17 @@ class Server:
             pass
 
 
+def request_bytes(n, p):
+    """Device bytes one request of block size n and p holds while it runs
+    (G, tau, field snapshot, plan workspace): what admission counts against
+    the HBM budget."""
+    out = C.c_size_t(0)
+    rc = lib.bsofi_server_request_bytes(n, p, C.byref(out))
+    if rc:
+        raise ValueError(f"request_bytes({n}, {p}) failed ({rc})")
+    return out.value
+
+
 def c5_chains(nchains=64, seed=20261203):
     """Chain sizes of the serving workload: n in {64, 144, 256} (lattice
     sides 8, 12, 16), p in {20, 40, 80}, drawn from a seeded generator."""

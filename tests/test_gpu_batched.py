@@ -44,9 +44,10 @@ def from_dev(dA, N):
     return dA.cpu().numpy().T[:N, :N]
 
 
-# (lattice side, p): n from 1 to 144, so that reflector steps of m <= 256
-# (4-wave kernel) and m > 256 (8-wave kernel) share rounds
-MIX = [(2, 3), (3, 7), (6, 5), (8, 4), (4, 12), (10, 3), (1, 9), (12, 3)]
+# (lattice side, p): n from 1 to 256, so that reflector steps of one to four
+# row tiles per wave share rounds, and the n = 256 matrix's 8-row-group panels
+# and 4-tile reflector steps run beside small matrices
+MIX = [(2, 3), (3, 7), (6, 5), (8, 4), (4, 12), (10, 3), (1, 9), (12, 3), (16, 3)]
 
 
 def test_batched_mix_equals_alone_and_oracle(B, eps):
