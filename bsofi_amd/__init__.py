@@ -35,7 +35,7 @@ BSOFI_ERR_NO_DEVICE = -1000
 BSOFI_ERR_ALLOC = -1001
 BSOFI_ERR_UNSUPPORTED = -1002
 BSOFI_ERR_HANDLE = -1003
-BSOFI_MAX_N = 512
+BSOFI_MAX_N = 1024
 
 PHASE_BSOF, PHASE_BSTRI, PHASE_BSOI, PHASE_ALL, PHASE_DIAG_ONLY = 1, 2, 4, 7, 8
 FAMILY = {"panel": 0, "refl": 1, "gemm": 2, "trtri": 3, "eye": 4, "scan": 5, "larft": 6}

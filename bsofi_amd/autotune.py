@@ -4,8 +4,8 @@ For every block size n it times, on the device, a small batch of seeded
 DQMC matrices of that n under each supported pair
 
     nb       -- sub-panel width of BSOF / BSOI (16 or 32 where 2n <= 512,
-                16 where 2n <= 1024): narrower panels shorten each column
-                step of the panel QR but add reflector launches;
+                else 8 or 16): narrower panels shorten each column step of
+                the panel QR but add reflector launches;
     scan_mti -- rows per workgroup of the BSOI scan, 16 * scan_mti (1, 2, 4):
                 taller tiles reuse each Q_k fragment over more rows but
                 leave fewer workgroups for small batches,
@@ -25,11 +25,13 @@ from .workload import DeviceMatrices
 
 
 def candidates(n):
-    nbs = [nb for nb, ok in ((32, 2 * n <= 512), (16, 2 * n <= 1024), (8, True)) if ok][:2]
-    # scan forms: 16 * m rows on the 16x16x4 MFMA, or 9: 32 rows on the 4x4x4 MFMA
+    nbs = [nb for nb, ok in ((32, 2 * n <= 512), (16, True), (8, True)) if ok][:2]
+    # scan forms: 16 * m rows on the 16x16x4 MFMA with 4 waves (m = 1, 2, 4),
+    # 8: 16 rows with 8 waves (n > 512), or 9: 32 rows on the 4x4x4 MFMA
     # (from n = 128: below, its one workgroup per CU starves real batches,
     # which the tuner's small trial batch does not show)
-    mtis = [m for m, ok in ((1, True), (2, n <= 256), (4, n <= 128), (9, 128 <= n <= 256 and n % 8 == 0)) if ok]
+    mtis = [m for m, ok in ((1, n <= 512), (2, n <= 256), (4, n <= 128), (8, n > 512),
+                            (9, 128 <= n <= 256 and n % 8 == 0)) if ok]
     return [(nb, m) for nb in nbs for m in mtis]
 
 

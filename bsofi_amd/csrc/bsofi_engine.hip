@@ -70,7 +70,7 @@ struct Geo {
   int scan_mti, scan_ntw;
 };
 
-static int nb_for(int n) { return (2 * n <= 512) ? 32 : (2 * n <= 1024 ? 16 : 8); }
+static int nb_for(int n) { return (2 * n <= 512) ? 32 : 16; }
 
 // Choices a context can override per block size n (the autotuner's knobs):
 // the sub-panel width nb of BSOF / BSOI and the rows per workgroup of the
@@ -83,22 +83,29 @@ struct Tune {
 static bool nb_supported(int n, int nb) {
   const int m = 2 * n;
   if (nb == 32) return m <= 512;
-  if (nb == 16) return m <= 1024;
+  if (nb == 16) return m <= 2048;
   if (nb == 8) return m <= 2048;
   return false;
 }
 
-static int scan_ntw_for(int n) { return ((2 * n + 15) / 16 + 3) / 4; }
-
-// Scan forms: scan_mti = 1, 2, 4 is bsoi_scan_kernel (16x16x4 MFMA) on
-// 16 * scan_mti rows; kScan4 is bsoi_scan4_kernel (4x4x4 MFMA) on 32 rows.
+// Scan forms: scan_mti = 1, 2, 4 is bsoi_scan_kernel (16x16x4 MFMA, 4 waves)
+// on 16 * scan_mti rows; kScan8W is the same kernel on 16 rows with 8 waves
+// (block sizes 512 < n <= 1024: 16 output tiles per wave instead of 32);
+// kScan4 is bsoi_scan4_kernel (4x4x4 MFMA) on 32 rows.
+static constexpr int kScan8W = 8;
 static constexpr int kScan4 = 9;
-static int scan_rows(int mti) { return mti == kScan4 ? 32 : 16 * mti; }
+static int scan_rows(int mti) { return mti == kScan4 ? 32 : (mti == kScan8W ? 16 : 16 * mti); }
+static int scan_waves(int mti) { return mti == kScan8W ? 8 : 4; }
+static int scan_ntw_for(int n, int mti) {
+  const int w = scan_waves(mti);
+  return ((2 * n + 15) / 16 + w - 1) / w;
+}
 
 static bool scan_variant_ok(int mti, int ntw) {
   // with 8 | n (scan_ok); ntw <= 8 means n <= 256, the column count of the
   // kernel's LDS staging buffer for X_{r,k}
   if (mti == kScan4) return ntw >= 1 && ntw <= 8;
+  if (mti == kScan8W) return ntw >= 9 && ntw <= 16;
   if (mti == 1) return ntw >= 1 && ntw <= 16;
   if (mti == 2) return ntw >= 1 && ntw <= 8;
   if (mti == 4) return ntw >= 1 && ntw <= 4;
@@ -120,18 +127,22 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   // panel_factor_kernel<RPG, CPW, NW>: 8 waves of 4 columns for every
   // 32-wide sub-panel of 2n <= 512, so that a batch round of such matrices
   // takes one panel launch (over 512 rows here; the round's largest m picks
-  // 128 or 256 where it can); else 4 waves of nb / 4 columns over the next
-  // power-of-two row count
-  const bool p8 = g.nb == 32 && m <= 512;
-  const int rpg = p8 ? 8 : m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
+  // 128 or 256 where it can); 8 waves of 2 columns for 16-wide sub-panels
+  // of 2n > 1024 (32 rows per lane: 4 waves of 4 columns would not fit the
+  // registers); else 4 waves of nb / 4 columns over the next power-of-two
+  // row count
+  const bool p32 = g.nb == 32 && m <= 512;
+  const bool p8 = p32 || (g.nb == 16 && m > 1024);
+  const int rpg = p32 ? 8 : m <= 128 ? 2 : m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : 32;
   const int nw = p8 ? 8 : 4;
   g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
   // default scan form: the 4x4x4 scan from n = 128 (one workgroup per CU;
   // below that its k-loops are short and batches of small matrices need the
-  // occupancy of the 16x16x4 form)
-  g.scan_mti = (tune && tune->scan_mti) ? tune->scan_mti
-                                         : (n <= 256 ? (n >= 128 && n % 8 == 0 ? kScan4 : 2) : 1);
-  g.scan_ntw = scan_ntw_for(n);
+  // occupancy of the 16x16x4 form); 8 waves beyond n = 512
+  g.scan_mti = (tune && tune->scan_mti)
+                   ? tune->scan_mti
+                   : (n <= 256 ? (n >= 128 && n % 8 == 0 ? kScan4 : 2) : (n <= 512 ? 1 : kScan8W));
+  g.scan_ntw = scan_ntw_for(n, g.scan_mti);
   return g;
 }
 
@@ -319,7 +330,8 @@ struct Planner {
 
   // refl_apply_kernel instance for the largest m: 8 waves with the fewest
   // row tiles each that cover it, RT = 1 .. 4, or 8 beyond m = 512 (variant =
-  // 100 + RT, m <= 128 RT).  Every RT sums a task's tiles on the same chains
+  // 100 + RT, m <= 128 RT); beyond m = 1024 refl_apply_stream_kernel
+  // (variant 116), which holds no tile across its phases.  Every RT sums a task's tiles on the same chains
   // in the same order.  Below m = 256 the 8-wave instances replaced 4-wave
   // ones with twice the tiles per wave, half the MFMA chain per wave (the
   // uneven matrices one at a time +1.9 %, batches unchanged); RT = 3 for
@@ -327,7 +339,7 @@ struct Planner {
   static int refl_variant(const std::vector<ReflTask>& rt) {
     int mm = 0;
     for (auto& q : rt) mm = std::max(mm, q.m);
-    return mm <= 512 ? 100 + std::max(1, (mm + 127) / 128) : 108;
+    return mm <= 512 ? 100 + std::max(1, (mm + 127) / 128) : (mm <= 1024 ? 108 : 116);
   }
 
   // Reflector tasks of one launch step.  The kernel instance follows from
@@ -758,17 +770,20 @@ struct Planner {
 // ---------------------------------------------------------------------------
 // Executor
 // ---------------------------------------------------------------------------
+// scan_mti kScan8W is bsoi_scan_kernel<1, NTW, 8>
 template <int MTI, int NTW>
 static void launch_scan(const Launch& L, const char* base, hipStream_t s) {
-  hipLaunchKernelGGL((bsoi_scan_kernel<MTI, NTW>), dim3(L.grid), dim3(256), L.shmem, s,
+  constexpr int M = MTI == kScan8W ? 1 : MTI, NW = MTI == kScan8W ? 8 : 4;
+  hipLaunchKernelGGL((bsoi_scan_kernel<M, NTW, NW>), dim3(L.grid), dim3(64 * NW), L.shmem, s,
                      (const ScanTask*)(base + L.task_off), (const Item*)(base + L.item_off), L.ldcar);
 }
 
 template <int MTI, int NTW>
 static void set_scan_attr(size_t shmem) {
+  constexpr int M = MTI == kScan8W ? 1 : MTI, NW = MTI == kScan8W ? 8 : 4;
   if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)bsoi_scan_kernel<MTI, NTW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
+    (void)hipFuncSetAttribute((const void*)bsoi_scan_kernel<M, NTW, NW>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
 }
 
 template <int W>
@@ -806,6 +821,9 @@ static void set_scan_attr4(size_t shmem) {
     SCAN_C(FN, 2, 7, __VA_ARGS__) SCAN_C(FN, 2, 8, __VA_ARGS__)                                              \
     SCAN_C(FN, 4, 1, __VA_ARGS__) SCAN_C(FN, 4, 2, __VA_ARGS__) SCAN_C(FN, 4, 3, __VA_ARGS__)                \
     SCAN_C(FN, 4, 4, __VA_ARGS__)                                                                            \
+    SCAN_C(FN, 8, 9, __VA_ARGS__) SCAN_C(FN, 8, 10, __VA_ARGS__) SCAN_C(FN, 8, 11, __VA_ARGS__)              \
+    SCAN_C(FN, 8, 12, __VA_ARGS__) SCAN_C(FN, 8, 13, __VA_ARGS__) SCAN_C(FN, 8, 14, __VA_ARGS__)             \
+    SCAN_C(FN, 8, 15, __VA_ARGS__) SCAN_C(FN, 8, 16, __VA_ARGS__)                                            \
     default: break;                                                                                          \
   }
 
@@ -824,7 +842,7 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
         PANEL_CASE(2, 4, 4) PANEL_CASE(4, 4, 4) PANEL_CASE(8, 4, 4) PANEL_CASE(16, 4, 4)
         PANEL_CASE(2, 2, 4) PANEL_CASE(4, 2, 4) PANEL_CASE(8, 2, 4) PANEL_CASE(16, 2, 4) PANEL_CASE(32, 2, 4)
         PANEL_CASE(2, 4, 8) PANEL_CASE(3, 4, 8) PANEL_CASE(4, 4, 8) PANEL_CASE(5, 4, 8) PANEL_CASE(6, 4, 8)
-        PANEL_CASE(7, 4, 8) PANEL_CASE(8, 4, 8)
+        PANEL_CASE(7, 4, 8) PANEL_CASE(8, 4, 8) PANEL_CASE(32, 2, 8)
         default: break;
       }
 #undef PANEL_CASE
@@ -850,6 +868,7 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
         case 204:
           hipLaunchKernelGGL((refl_apply_kernel<4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
           break;
+        case 116: hipLaunchKernelGGL(refl_apply_stream_kernel, dim3(L.grid), dim3(512), 0, s, t, it); break;
         default: hipLaunchKernelGGL((refl_apply_kernel<8, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
       }
       break;
@@ -885,6 +904,28 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     }
     default:
       break;
+  }
+}
+
+// Whether run_launch has a kernel instance for a launch: a plan whose
+// launches it could not issue is refused at creation instead of silently
+// skipping work.
+static bool known_variant(const Launch& L) {
+  switch (L.family) {
+    case F_PANEL: {
+      static const int ok[] = {40204, 40404, 40804, 41604, 40202, 40402, 40802, 41602, 43202, 80204, 80304,
+                               80404, 80504, 80604, 80704, 80804, 83202};
+      return std::find(std::begin(ok), std::end(ok), L.variant) != std::end(ok);
+    }
+    case F_REFL:
+      return (L.variant >= 101 && L.variant <= 104) || (L.variant >= 201 && L.variant <= 204) || L.variant == 108 ||
+             L.variant == 116;
+    case F_GEMM:
+      return L.variant >= 0 && L.variant <= 2;
+    case F_SCAN:
+      return scan_variant_ok(L.variant / 100, L.variant % 100);
+    default:
+      return true;
   }
 }
 
@@ -1146,6 +1187,14 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     }
   }
   for (auto& L : P->launches) {
+    if (!known_variant(L)) {
+      fprintf(stderr, "bsofi_amd: no kernel instance for family %d variant %d\n", L.family, L.variant);
+      if (P->fork) (void)hipEventDestroy(P->fork);
+      if (P->join) (void)hipEventDestroy(P->join);
+      if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
+      delete P;
+      return BSOFI_ERR_UNSUPPORTED;
+    }
     if (L.family == F_SCAN) {
       const int variant = L.variant;
       SCAN_SWITCH(set_scan_attr, L.shmem)

@@ -703,6 +703,126 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
   }
 }
 
+// Reflector application for slabs taller than the register-resident forms
+// hold (m > 1024, block sizes n > 512): the same three phases and the same
+// summation order as refl_apply_kernel with 8 waves (wave w takes row tiles
+// w, w + 8, ... in increasing order into one chain per 16 columns of W), but
+// no tile is held between phases: phase 1 streams the slab's row tiles
+// through the accumulator layout, phase 3 loads each tile again, adds V Wf
+// and stores it.  A task therefore gets the bits it gets from the resident
+// form at any m, for twice the slab's read traffic.
+__global__ __launch_bounds__(512) void refl_apply_stream_kernel(const ReflTask* __restrict__ tasks,
+                                                                const Item* __restrict__ items) {
+  constexpr int NW = 8, NT = 512;
+  const Item it = items[blockIdx.x];
+  const ReflTask t = tasks[it.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m = t.m, nbj = t.nbj;
+
+  __shared__ double Wf[32][17];
+  __shared__ double Tl[32][33];
+  __shared__ double XW[NW][32 * 17];
+  __shared__ double W[32][17];
+  auto Wp = [&](int w) { return reinterpret_cast<double (*)[17]>(XW[w]); };
+
+  gptr<double> cp[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int vcol = it.sub * 16 + 4 * r + lk;
+    cp[r] = nullptr;
+    if (vcol < t.w1)
+      cp[r] = glb(t.C1) + (size_t)vcol * t.ldc;
+    else if (vcol - t.w1 < t.w2)
+      cp[r] = glb(t.C2) + (size_t)(vcol - t.w1) * t.ldc;
+  }
+  const gptr<const double> Vg = glb(t.V);
+  for (int e = tid; e < 32 * 32; e += NT) {
+    const int i = e & 31, j = e >> 5;
+    Tl[i][j] = (i < nbj && j < nbj) ? glb(t.T)[(size_t)j * t.ldt + i] : 0.0;
+  }
+  double (*X)[33] = reinterpret_cast<double (*)[33]>(XW[wave]);
+  const bool two = nbj > 16;
+  // one row tile of the slab into the accumulator layout C[4r + lk][li]
+  auto load_tile = [&](int rt, d4& acc) {
+    const int row = rt * 16 + li;
+    double cl[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cl[r] = (cp[r] && row < m) ? cp[r][row] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[li][4 * r + lk] = cl[r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = X[4 * r + lk][li];
+  };
+
+  // phase 1: W = V^T C, wave w's chain over tiles w, w + 8, ...
+  d4 w0 = d4zero(), w1 = d4zero();
+  for (int rt = wave; rt * 16 < m; rt += NW) {
+    d4 acc;
+    load_tile(rt, acc);
+    const int row = rt * 16 + li;
+#pragma unroll
+    for (int kq = 0; kq < 8; ++kq) X[li][4 * kq + lk] = vload(Vg, t.ldv, m, nbj, row, 4 * kq + lk);
+    double va[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      va[r][0] = X[4 * r + lk][li];
+      va[r][1] = X[4 * r + lk][16 + li];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      w0 = mfma(va[r][0], acc[r], w0);
+      if (two) w1 = mfma(va[r][1], acc[r], w1);
+    }
+  }
+  {
+    double (*Wq)[17] = Wp(wave);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Wq[crow(lk, r)][li] = w0[r];
+      Wq[16 + crow(lk, r)][li] = two ? w1[r] : 0.0;
+    }
+  }
+  __syncthreads();
+  // phase 2: Wf = -op(T) W, partial chains summed as refl_apply_kernel's
+  for (int e = tid; e < 32 * 16; e += NT) {
+    const int i = e >> 4, j = e & 15;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v += Wp(q)[i][j] + Wp(q + 4)[i][j];
+    W[i][j] = v;
+  }
+  __syncthreads();
+  if (wave < 2) {
+    const int i = wave * 16 + li;
+    d4 tw = d4zero();
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) {
+      const int k = 4 * kc + lk;
+      tw = mfma(t.trans ? Tl[k][i] : Tl[i][k], W[k][li], tw);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
+  }
+  __syncthreads();
+  // phase 3: C += V Wf tile by tile (each tile loaded again)
+  for (int rt = wave; rt * 16 < m; rt += NW) {
+    d4 acc;
+    load_tile(rt, acc);
+    const int row = rt * 16 + li;
+#pragma unroll
+    for (int kq = 0; kq < 8; ++kq)
+      if (4 * kq < nbj) acc = mfma(vload(Vg, t.ldv, m, nbj, row, 4 * kq + lk), Wf[4 * kq + lk][li], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[crow(lk, r)][li] = acc[r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double v = X[li][4 * r + lk];
+      if (cp[r] && row < m) cp[r][row] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // T factor from stored reflectors and tau (dlarft), one workgroup per
 // sub-panel: G = V^T V on MFMA (K = m split over the waves), then the
@@ -963,14 +1083,15 @@ __global__ void eye_kernel(const EyeTask* __restrict__ tasks, int ntasks) {
 // so only the carry half (K = n) is multiplied; the diagonal block X_{k,k}
 // is read through an upper-triangular mask because A still holds the
 // reflectors below its diagonal.  The 2n output columns are split in
-// 16-column tiles dealt round-robin to the 4 waves (NTW tiles per wave).
+// 16-column tiles dealt round-robin to the NW waves (NTW tiles per wave).
 // With diag_only the workgroup stops once its rows' diagonal blocks are
 // done (block row r is finished at k = r - 1) and writes only G_rr; the
 // diagonal blocks are the same bits as in the full inverse.
-template <int MTI, int NTW>
-__global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restrict__ tasks,
-                                                        const Item* __restrict__ items, int ldcar) {
+template <int MTI, int NTW, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void bsoi_scan_kernel(const ScanTask* __restrict__ tasks,
+                                                            const Item* __restrict__ items, int ldcar) {
   constexpr int MT = 16 * MTI;
+  constexpr int NT = 64 * NW;
   extern __shared__ double carry[];  // carry[kk * ldcar + i], kk < n, i < MT
   const Item it = items[blockIdx.x];
   const ScanTask t = tasks[it.task];
@@ -990,7 +1111,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   }
   const int ntiles = (n2 + 15) / 16;
 
-  for (int e = tid; e < n * MT; e += 256) {
+  for (int e = tid; e < n * MT; e += NT) {
     const int i = e % MT, kk = e / MT;
     const int row = R0 + i;
     double v = 0.0;
@@ -1007,7 +1128,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   const int kstop = diag ? max(rbmin - 1, 0) : 0;
   int bcol[NTW];
 #pragma unroll
-  for (int b = 0; b < NTW; ++b) bcol[b] = (wave + 4 * b) * 16 + li;
+  for (int b = 0; b < NTW; ++b) bcol[b] = (wave + NW * b) * 16 + li;
   for (int k = p - 2; k >= kstop; --k) {
     const gptr<const double> Qk = glb(t.Q) + (size_t)k * t.qstride;
     d4 acc[MTI][NTW];
@@ -1017,12 +1138,43 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
       for (int b = 0; b < NTW; ++b) acc[a][b] = d4zero();
 
     // Fragments of the next k-step are loaded while the MFMAs of the
-    // current one run (two register buffers, ping-pong).
-    double a0[MTI], b0[NTW], a1[MTI], b1[NTW];
+    // current one run (two register buffers, ping-pong).  The 8-wave form
+    // (16 tiles per wave, two waves per SIMD) has no registers for the second
+    // buffer and loads each k-step's fragments right before its MFMAs; the
+    // k order, and so every sum, is the same.
+    constexpr bool DB = NW == 4;
+    double a0[MTI], b0[NTW], a1[DB ? MTI : 1], b1[DB ? NTW : 1];
+    auto run_part = [&](auto load) {
+      if constexpr (DB) {
+        load(a0, b0, 0);
+        for (int k0 = 0; k0 < n; k0 += 8) {
+          if (k0 + 4 < n) load(a1, b1, k0 + 4);
+#pragma unroll
+          for (int a = 0; a < MTI; ++a)
+#pragma unroll
+            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
+          if (k0 + 8 < n) load(a0, b0, k0 + 8);
+          if (k0 + 4 < n) {
+#pragma unroll
+            for (int a = 0; a < MTI; ++a)
+#pragma unroll
+              for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a1[a], b1[b], acc[a][b]);
+          }
+        }
+      } else {
+        for (int k0 = 0; k0 < n; k0 += 4) {
+          load(a0, b0, k0);
+#pragma unroll
+          for (int a = 0; a < MTI; ++a)
+#pragma unroll
+            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
+        }
+      }
+    };
     if (rbmin <= k) {
       // [X_{r,k}] part: K rows 0..n-1 of Q_k^T
       const gptr<const double> Xk = A + (size_t)k * n * lda;
-      auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
+      auto load = [&](auto& af, auto& bf, int k0) {
         const int kk = k0 + lk;
 #pragma unroll
         for (int a = 0; a < MTI; ++a) {
@@ -1033,53 +1185,25 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
 #pragma unroll
         for (int b = 0; b < NTW; ++b) bf[b] = (kk < n && bcol[b] < n2) ? Qk[(size_t)kk * n2 + bcol[b]] : 0.0;
       };
-      load(a0, b0, 0);
-      for (int k0 = 0; k0 < n; k0 += 8) {
-        if (k0 + 4 < n) load(a1, b1, k0 + 4);
-#pragma unroll
-        for (int a = 0; a < MTI; ++a)
-#pragma unroll
-          for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
-        if (k0 + 8 < n) load(a0, b0, k0 + 8);
-        if (k0 + 4 < n) {
-#pragma unroll
-          for (int a = 0; a < MTI; ++a)
-#pragma unroll
-            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a1[a], b1[b], acc[a][b]);
-        }
-      }
+      run_part(load);
     }
     // [carry] part: K rows n..2n-1 of Q_k^T
     {
       const gptr<const double> Qc = Qk + (size_t)n * n2;
-      auto load = [&](double (&af)[MTI], double (&bf)[NTW], int k0) {
+      auto load = [&](auto& af, auto& bf, int k0) {
         const int kk = k0 + lk;
 #pragma unroll
         for (int a = 0; a < MTI; ++a) af[a] = (kk < n) ? carry[kk * ldcar + a * 16 + li] : 0.0;
 #pragma unroll
         for (int b = 0; b < NTW; ++b) bf[b] = (kk < n && bcol[b] < n2) ? Qc[(size_t)kk * n2 + bcol[b]] : 0.0;
       };
-      load(a0, b0, 0);
-      for (int k0 = 0; k0 < n; k0 += 8) {
-        if (k0 + 4 < n) load(a1, b1, k0 + 4);
-#pragma unroll
-        for (int a = 0; a < MTI; ++a)
-#pragma unroll
-          for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a0[a], b0[b], acc[a][b]);
-        if (k0 + 8 < n) load(a0, b0, k0 + 8);
-        if (k0 + 4 < n) {
-#pragma unroll
-          for (int a = 0; a < MTI; ++a)
-#pragma unroll
-            for (int b = 0; b < NTW; ++b) acc[a][b] = mfma(a1[a], b1[b], acc[a][b]);
-        }
-      }
+      run_part(load);
     }
     __syncthreads();
     // epilogue
 #pragma unroll
     for (int b = 0; b < NTW; ++b) {
-      const int tile = wave + 4 * b;
+      const int tile = wave + NW * b;
       if (tile >= ntiles) continue;
       const int col = tile * 16 + li;
       if (col >= n2) continue;
@@ -1102,7 +1226,7 @@ __global__ __launch_bounds__(256) void bsoi_scan_kernel(const ScanTask* __restri
   }
   // block column 0 (the carry after k = 0), for diag_only only its diagonal
   if (!diag || rbmin == 0) {
-    for (int e = tid; e < n * MT; e += 256) {
+    for (int e = tid; e < n * MT; e += NT) {
       const int i = e % MT, kk = e / MT;
       const int row = R0 + i;
       if (row < N && (!diag || row < n)) A[(size_t)kk * lda + row] = carry[kk * ldcar + i];

@@ -166,13 +166,14 @@ void* bsofi_ctx_stream(bsofi_ctx_t ctx);
 int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
 /* Tuning (the autotuner's knobs), per block size n, for plans created on
  * the context afterwards: nb = sub-panel width of BSOF/BSOI (8, 16 or 32;
- * 32 needs 2n <= 512, 16 needs 2n <= 1024), scan_mti = form of the BSOI
- * scan: 1, 2 or 4 = 16-row multiples per workgroup on the 16x16x4 FP64
- * MFMA (4 needs n <= 128, 2 needs n <= 256), 9 = 32 rows on the 4x4x4
- * FP64 MFMA (needs n <= 256 and n % 8 == 0; the default for n >= 128).  0 keeps
- * the default; unsupported pairs return
- * BSOFI_ERR_UNSUPPORTED.  No counterpart in the reference, whose static
- * CPU/GPU split (KAPPA_*, *_SWITCH_*) these choices replace. */
+ * 32 needs 2n <= 512), scan_mti = form of the BSOI scan: 1, 2 or 4 =
+ * 16-row multiples per workgroup on the 16x16x4 FP64 MFMA with 4 waves
+ * (1 needs n <= 512, 2 needs n <= 256, 4 needs n <= 128), 8 = 16 rows with
+ * 8 waves (512 < n <= 1024, the default there), 9 = 32 rows on the 4x4x4
+ * FP64 MFMA (needs n <= 256 and n % 8 == 0; the default for n >= 128).  0
+ * keeps the default; unsupported pairs return BSOFI_ERR_UNSUPPORTED.  No
+ * counterpart in the reference, whose static CPU/GPU split (KAPPA_*,
+ * *_SWITCH_*) these choices replace. */
 int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti);
 int bsofi_ctx_get_tuning(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti);
 int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
@@ -197,7 +198,7 @@ int bsofi_ctx_release_cache(bsofi_ctx_t ctx);
  * figures next to the datasheet peak.  No counterpart in the reference. */
 int bsofi_fp64_rate(bsofi_ctx_t ctx, int kind, double* tflops);
 /* Supported block sizes: 1 <= n <= BSOFI_MAX_N, L >= 2. */
-#define BSOFI_MAX_N 512
+#define BSOFI_MAX_N 1024
 int bsofi_check_args(int n, int L, int lda);
 
 /* Device workspace (bytes) one matrix of order (n, p) needs. */

@@ -351,15 +351,16 @@ def test_serving_alone_equals_mix(B, eps):
 def test_tuned_variants_match_oracle(B, eps):
     """Every (nb, scan_mti) pair the autotuner may pick gives the oracle's
     inverse: the nb = 16 panel instances, the 64-row scan tiles, the 16-row
-    scan tiles below n = 256, and the 32-row 4x4x4 scan (scan_mti 9) where
-    8 divides n (from n = 128)."""
+    scan tiles below n = 256, the 32-row 4x4x4 scan (scan_mti 9) where
+    8 divides n (from n = 128), and for n = 576 both sub-panel widths (8 and
+    16) under the 8-wave scan."""
     import torch
 
     from bsofi_amd import autotune
 
     O = oracle.Oracle("auto", threads=1)
     ctx = B.Context(0)
-    for lx, p in [(4, 5), (8, 4), (10, 3), (12, 3)]:
+    for lx, p in [(4, 5), (8, 4), (10, 3), (12, 3), (24, 2)]:
         n = lx * lx
         A0, _ = dqmc.dqmc_matrix(lx, p, instance=70 + lx)
         G = A0.copy(order="F")

@@ -319,3 +319,93 @@ def test_c5_serving_budget_queue_and_alone(B, eps):
     ctx.close()
     assert got is not None and kept is not None
     assert np.array_equal(got, kept)
+
+
+def test_large_n_phases_match_oracle(B, O, eps):
+    """Block sizes above 512 (2n up to 2048): 16-wide sub-panels on the
+    8-wave panel instance with 32 rows per lane, the streaming reflector
+    kernel (m > 1024), the 8-wave 16-row scan.  Phase by phase against the
+    oracle: a random n = 600, p = 2 and a 32 x 32 DQMC lattice (n = 1024),
+    p = 3."""
+    for kind, a, p in [("rand_eye", 600, 2), ("dqmc", 32, 3)]:
+        if kind == "dqmc":
+            A0, m = dqmc.dqmc_matrix(a, p, instance=12)
+            n = m["n"]
+        else:
+            A0, n = dqmc.random_pcyclic(a, p, seed=13), a
+        N = n * p
+        print(f"case {kind} n={n} p={p}", flush=True)
+        Ao = A0.copy(order="F")
+        tau_o = np.zeros(N)
+        assert O.bsof(n, p, Ao, tau_o) == 0
+        Ag = A0.copy(order="F")
+        tau_g = np.zeros(N)
+        assert B.lapackXbsof(n, p, Ag, tau_g) == 0
+        assert rel(tau_g, tau_o) <= 1e-10
+        assert rel(Ag, Ao) <= 1e-10
+        Xg = Ao.copy(order="F")
+        assert B.lapackXbstri(n, p, Xg) == 0
+        assert O.bstri(n, p, Ao) == 0
+        assert rel(Xg, Ao) <= 1e-10
+        Go = Ao.copy(order="F")
+        assert O.bsoi(n, p, Go, tau_o) == 0
+        assert B.lapackXbsoi_ormqr(n, p, Ao, tau_o.copy()) == 0
+        assert rel(Ao, Go) <= 1e-10
+        d, o = oracle.pcyclic_blocks(A0, n, p)
+        assert oracle.residual_blocks(d, o, Ao, n, p) <= 100 * N * eps
+
+
+@pytest.mark.parametrize("p", [3, 20])
+def test_n1024_through_handle_api(B, O, eps, p):
+    """The reference's own Fortran example size (n = 1024, L = 20,
+    examples/ftest_hybrid/ftest_hybrid.f90:215-216) through hybridxinith_ ..
+    hybridxfinalizeh_: residual <= 100 N eps, and at p = 3 (beta = 0.375) G
+    within 1e-10 of the oracle's."""
+    A0, m = dqmc.dqmc_matrix(32, p, instance=1)
+    n, N = m["n"], m["n"] * p
+    print(f"case handle API n={n} p={p} N={N}", flush=True)
+    A = A0.copy(order="F")
+    tau = np.zeros(N)
+    h = B.HandleAPI(n, p, N)
+    assert h.rc == 0
+    work = np.zeros(max(h.lwork(), 1))
+    assert h.bsoftri(A, tau, work) == 0
+    assert h.bsoi(A, tau, work) == 0
+    assert h.finalize() == 0
+    d, o = oracle.pcyclic_blocks(A0, n, p)
+    res = oracle.residual_blocks(d, o, A, n, p)
+    print(f"  residual {res:.3e} (bound {100 * N * eps:.3e})", flush=True)
+    assert res <= 100 * N * eps
+    if p * 0.125 <= 4:
+        G = A0.copy(order="F")
+        O.invert(n, p, G)
+        assert rel(A, G) <= 1e-10
+
+
+def test_large_n_in_a_mix_same_bits(B, eps):
+    """An n = 1024 matrix in one plan with small ones: its steps with m >
+    1024 put every matrix of the round on the streaming reflector kernel,
+    which sums in the resident kernels' order, so each matrix gets the bits
+    it gets alone."""
+    import torch
+
+    ctx = B.Context(0)
+    mix = [(32, 2), (3, 5), (8, 4), (12, 3), (16, 2)]
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=40 + k)[0] for k, (lx, p) in enumerate(mix)]
+    dims = [(lx * lx, p) for lx, p in mix]
+    devs = [to_dev(A) for A in inputs]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in dims]
+    torch.cuda.synchronize()
+    assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)
+                                  for d, t, (n, p) in zip(devs, taus, dims)]) == [0] * len(mix)
+    for k, (A0, (n, p)) in enumerate(zip(inputs, dims)):
+        print(f"mix case {k}: n={n} p={p}", flush=True)
+        d = to_dev(A0)
+        t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+        alone = from_dev(d, n * p)
+        assert np.array_equal(alone, from_dev(devs[k], n * p)), f"case {k} differs between alone and mix"
+        dd, oo = oracle.pcyclic_blocks(A0, n, p)
+        assert oracle.residual_blocks(dd, oo, alone, n, p) <= 100 * n * p * eps
+    ctx.close()
