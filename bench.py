@@ -46,6 +46,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# BSOF schedule of every context the benchmark creates (bsofi_ctx_set_schedule):
+# 0 = one launch per sweep step (the library default), 1 = one device-scheduled launch per plan
+SCHEDULE = 0
+
+
+def new_ctx(B):
+    ctx = B.Context(0)
+    ctx.set_schedule(SCHEDULE)
+    return ctx
+
+
 def fp64_rates(B):
     """This GPU's FP64 rates for back-to-back instructions of one kind on
     every CU (bsofi_fp64_rate): the ceilings the kernels actually face."""
@@ -128,7 +139,7 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None):
     import torch
 
     phases = B.PHASE_ALL if phases is None else phases
-    ctx = B.Context(0)
+    ctx = new_ctx(B)
     if nb or scan_mti:
         ctx.set_tuning(lx * lx, nb, scan_mti)
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -182,7 +193,7 @@ def bench_batched(B, lx, p, count, reps, seed):
     rebuilt between executions, outside the timed interval)."""
     import torch
 
-    ctx = B.Context(0)
+    ctx = new_ctx(B)
     stream = torch.cuda.ExternalStream(ctx.stream)
     log(f"[bench] batched: building {count} matrices n={lx * lx} p={p}")
     ds = DeviceSet(B, ctx, lx, p, count, seed, first_instance=1000)
@@ -227,7 +238,7 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn):
     from bsofi_amd import autotune, flops, workload
 
     specs = workload.uneven_specs(count, seed)
-    ctx = B.Context(0)
+    ctx = new_ctx(B)
     t0 = time.perf_counter()
     table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=8, reps=2, log=log_fn)
     tune_s = time.perf_counter() - t0
@@ -256,7 +267,7 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn):
     batched_ms = float(np.median(times))
     # plain path: one plan per matrix on an untuned context
     mats.rebuild()
-    pctx = B.Context(0)
+    pctx = new_ctx(B)
     pstream = torch.cuda.ExternalStream(pctx.stream)
     plans = [B.Plan(pctx, [mats.desc(k)]) for k in range(len(mats))]
     plans[0].execute()  # warm-up of the kernels, then fresh input
@@ -390,6 +401,8 @@ def main():
     ap.add_argument("--diag", type=int, default=1, help="also time the diagonal-blocks-only mode on the same size")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--schedule", type=int, default=0,
+                    help="BSOF schedule: 1 = one device-scheduled launch per plan, 0 = one launch per sweep step")
     ap.add_argument("--nb", type=int, default=0, help="sub-panel width for the plain run (0 = default)")
     ap.add_argument("--scan-mti", type=int, default=0, help="scan tile height / 16 for the plain run (0 = default)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -417,6 +430,8 @@ def main():
     import bsofi_amd as B
     from bsofi_amd import flops
 
+    global SCHEDULE
+    SCHEDULE = args.schedule
     if B.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)

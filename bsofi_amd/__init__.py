@@ -35,10 +35,11 @@ BSOFI_ERR_NO_DEVICE = -1000
 BSOFI_ERR_ALLOC = -1001
 BSOFI_ERR_UNSUPPORTED = -1002
 BSOFI_ERR_HANDLE = -1003
+BSOFI_ERR_SCHEDULE = -1004
 BSOFI_MAX_N = 1024
 
 PHASE_BSOF, PHASE_BSTRI, PHASE_BSOI, PHASE_ALL, PHASE_DIAG_ONLY = 1, 2, 4, 7, 8
-FAMILY = {"panel": 0, "refl": 1, "gemm": 2, "trtri": 3, "eye": 4, "scan": 5, "larft": 6}
+FAMILY = {"panel": 0, "refl": 1, "gemm": 2, "trtri": 3, "eye": 4, "scan": 5, "larft": 6, "det": 8, "flow": 9}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -91,6 +92,8 @@ def _load():
         "bsofi_det": (i, [_vp, _vp, i, i, i, _vp, _dp, _dp]),
         "bsofi_ctx_cached_bytes": (C.c_size_t, [_vp]),
         "bsofi_ctx_set_lanes": (i, [_vp, i]),
+        "bsofi_ctx_set_schedule": (i, [_vp, i]),
+        "bsofi_ctx_set_wait_limit": (i, [_vp, d]),
         "bsofi_fp64_rate": (i, [_vp, i, _dp]),
         "bsofi_ctx_release_cache": (i, [_vp]),
         "bsofi_ctx_get_tuning": (i, [_vp, i, C.POINTER(i), C.POINTER(i)]),
@@ -265,6 +268,18 @@ class Context:
         rc = lib.bsofi_ctx_set_lanes(self.h, lanes)
         if rc:
             raise RuntimeError(f"bsofi_ctx_set_lanes failed ({rc})")
+
+    def set_schedule(self, mode):
+        """1: BSOF as one device-scheduled launch where supported, 0: one
+        launch per sweep step."""
+        rc = lib.bsofi_ctx_set_schedule(self.h, mode)
+        if rc:
+            raise ValueError(f"bsofi_ctx_set_schedule({mode}) failed ({rc})")
+
+    def set_wait_limit(self, seconds):
+        rc = lib.bsofi_ctx_set_wait_limit(self.h, float(seconds))
+        if rc:
+            raise ValueError(f"bsofi_ctx_set_wait_limit({seconds}) failed ({rc})")
 
     def synchronize(self):
         rc = lib.bsofi_ctx_synchronize(self.h)

@@ -43,13 +43,23 @@ bool hip_ok(hipError_t e, const char* what) {
   } while (0)
 
 enum Family { F_PANEL = 0, F_REFL = 1, F_GEMM = 2, F_TRTRI = 3, F_EYE = 4, F_SCAN = 5, F_LARFT = 6, F_FILL = 7,
-              F_DET = 8 };
+              F_DET = 8, F_FLOW = 9 };
 
 struct Launch {
   int family = 0, variant = 0, grid = 0, ntasks = 0, ldcar = 0;
   int lane = 0;  // stream of a two-lane batch plan: 0 the context's, 1 its side stream
   int spw = 1;   // reflector launches: slabs per workgroup
+  bool prefork = false;  // runs on the context's stream before a two-lane plan forks
   size_t task_off = 0, item_off = 0, shmem = 0;
+  // F_FLOW: blob offsets of the panel, reflector and matrix tables; the
+  // zero-filled control region (queue head, stop word, counters, flags);
+  // the plan's info words; the wait limit in wall-clock ticks
+  size_t off2 = 0, off3 = 0, off4 = 0;
+  int* ctl = nullptr;
+  long long fill_count = 0;
+  int* info = nullptr;
+  int count = 0;
+  long long wait_ticks = 0;
 };
 
 struct Blob {
@@ -186,6 +196,8 @@ struct bsofi_ctx_s {
   bool own_stream = false;
   hipStream_t side = nullptr;  // second lane of batch plans, created on first use
   int lanes = 2;               // lanes of batch plans created on the context
+  int schedule = 0;            // 1: BSOF as one device-scheduled launch where supported, 0: launch list
+  double wait_s = 4.0;         // wait limit of the device-scheduled launch, seconds
   std::map<int, bsofi::Tune> tune;  // per block size n
   // Device buffers of destroyed plans (task blobs, workspaces), reused by
   // later plans on this context so that steady-state plan churn calls
@@ -289,7 +301,12 @@ struct MatJob {
   int* info;
   double* det;  // sign, log|det|
   double *T, *Q, *W, *Z1, *Z2, *Y;
+  int *fctr, *fdone;  // device-scheduled BSOF: column counters, sub-panel flags
 };
+
+// Sub-panels of one matrix's BSOF sweep (an upper bound over nb >= 8, so
+// that workspace sizes do not depend on the tuning).
+static size_t flow_subpanels_max(int n, int p) { return (size_t)p * ((2 * (size_t)n + 7) / 8); }
 
 static double* blk(const MatJob& m, int i, int j) {
   return m.A + (size_t)m.g.n * ((size_t)j * m.g.lda + i);
@@ -446,6 +463,118 @@ struct Planner {
       for (auto& kv : pt) emit(F_PANEL, panel_instance(kv.first, kv.second), kv.second, pi[kv.first]);
       emit_refl(rr);
     }
+  }
+
+  // -- BSOF as one device-scheduled launch (bsof_flow_kernel) --------------
+  // Every matrix on the 32-wide 8-wave panel (2n <= 512), whose body the
+  // flow kernel runs.
+  static bool flow_ok(const std::vector<MatJob>& mats) {
+    for (const MatJob& M : mats)
+      if (M.g.nb != 32 || 2 * M.g.n > 512) return false;
+    return !mats.empty();
+  }
+
+  // Queue order, per matrix, in phases r = 0, 1, ...: the QR of sub-panel r,
+  // then the reflector slabs of sub-panel r-1 that sub-panel r+1 does not
+  // need, then those of sub-panel r that it does (its lookahead); the
+  // matrices' phases interleaved.  Every task's inputs come from tasks
+  // earlier in that order.
+  void flow_bsof(std::vector<MatJob>& mats, int* ctl, long long fill_count, int* info, int count,
+                 long long wait_ticks) {
+    struct Sub {
+      int off, nbj, pref, rref, slabs, col1, col2, w1, w2;
+    };
+    std::vector<FlowMat> fm;
+    std::vector<PanelTask> pts;
+    std::vector<ReflTask> rts;
+    std::vector<std::vector<Sub>> subs(mats.size());
+    int maxm = 0;
+    size_t maxG = 0;
+    for (size_t i = 0; i < mats.size(); ++i) {
+      MatJob& M = mats[i];
+      const Geo& g = M.g;
+      fm.push_back({M.fctr, M.fdone, g.n, g.p, g.nb, (g.n + g.nb - 1) / g.nb});
+      for (int k = 0; k <= g.p - 2; ++k) {
+        const int ncols = (k == g.p - 2) ? 2 * g.n : g.n;
+        for (int j = 0; j * g.nb < ncols; ++j) {
+          Sub sb{};
+          sb.off = k * g.n + j * g.nb;
+          const int m = 2 * g.n - j * g.nb;
+          sb.nbj = std::min(g.nb, ncols - j * g.nb);
+          double* P = M.A + (size_t)sb.off * g.lda + sb.off;
+          sb.pref = (int)pts.size();
+          pts.push_back(PanelTask{P, M.tau + sb.off, Tptr(M, k, j), M.info, g.lda, m, sb.nbj, g.nb});
+          sb.w1 = 2 * g.n - j * g.nb - sb.nbj;
+          sb.w2 = (k < g.p - 2) ? g.n : 0;
+          sb.col1 = sb.off + sb.nbj;
+          sb.col2 = (g.p - 1) * g.n;
+          sb.rref = -1;
+          if (sb.w1 + sb.w2 > 0) {
+            ReflTask q;
+            q.V = P;
+            q.T = Tptr(M, k, j);
+            q.C1 = M.A + (size_t)(sb.off + sb.nbj) * g.lda + sb.off;
+            q.C2 = sb.w2 ? M.A + (size_t)((g.p - 1) * g.n) * g.lda + sb.off : nullptr;
+            q.ldv = g.lda;
+            q.ldt = g.nb;
+            q.ldc = g.lda;
+            q.m = m;
+            q.nbj = sb.nbj;
+            q.w1 = sb.w1;
+            q.w2 = sb.w2;
+            q.trans = 1;
+            sb.rref = (int)rts.size();
+            rts.push_back(q);
+            sb.slabs = (sb.w1 + sb.w2 + 15) / 16;
+          }
+          maxm = std::max(maxm, m);
+          subs[i].push_back(sb);
+        }
+      }
+      maxG = std::max(maxG, subs[i].size());
+    }
+    // does slab sl of sub-panel s touch a column in [lo, hi)?
+    auto touches = [](const Sub& sb, int sl, int lo, int hi) {
+      for (int v = 16 * sl; v < 16 * sl + 16 && v < sb.w1 + sb.w2; ++v) {
+        const int c = v < sb.w1 ? sb.col1 + v : sb.col2 + (v - sb.w1);
+        if (c >= lo && c < hi) return true;
+      }
+      return false;
+    };
+    std::vector<FlowTask> ft;
+    auto refl_tasks = [&](size_t i, size_t r, bool crit) {
+      const std::vector<Sub>& S = subs[i];
+      const Sub& sb = S[r];
+      if (sb.rref < 0) return;
+      const bool has_next = r + 1 < S.size();
+      const int lo = has_next ? S[r + 1].off : 0, hi = has_next ? S[r + 1].off + S[r + 1].nbj : 0;
+      for (int sl = 0; sl < sb.slabs; ++sl)
+        if ((has_next && touches(sb, sl, lo, hi)) == crit)
+          ft.push_back(FlowTask{1, (int)i, (int)r, sb.rref, sl, sb.col1, sb.col2, 0});
+    };
+    for (size_t r = 0; r <= maxG; ++r)
+      for (size_t i = 0; i < mats.size(); ++i) {
+        const size_t G = subs[i].size();
+        if (r < G) ft.push_back(FlowTask{0, (int)i, (int)r, subs[i][r].pref, 0, subs[i][r].off, 0, 0});
+        if (r >= 1 && r - 1 < G) refl_tasks(i, r - 1, false);
+        if (r < G) refl_tasks(i, r, true);
+      }
+    Launch L;
+    L.family = F_FLOW;
+    L.variant = maxm <= 128 ? 21 : (maxm <= 256 ? 42 : 84);  // bsof_flow_kernel<RPG, RT>
+    L.task_off = blob.push(ft);
+    L.off2 = blob.push(pts);
+    L.off3 = blob.push(rts);
+    L.off4 = blob.push(fm);
+    L.ntasks = (int)ft.size();
+    L.grid = (int)std::min<size_t>(ft.size(), cus);
+    L.prefork = true;
+    L.ctl = ctl;
+    L.fill_count = fill_count;
+    L.info = info;
+    L.count = count;
+    L.wait_ticks = wait_ticks;
+    push(L);
   }
 
   // The 8-wave panel instance holds 64 * RPG rows per column; a round takes
@@ -902,6 +1031,27 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       SCAN_SWITCH(launch_scan, L, base, s)
       break;
     }
+    case F_FLOW: {
+      const unsigned fill_blocks = (unsigned)std::min<long long>((L.fill_count + 255) / 256, 1 << 20);
+      hipLaunchKernelGGL(fill_int_kernel, dim3(fill_blocks), dim3(256), 0, s, L.ctl, (int)L.fill_count, 0);
+      auto ft = (const FlowTask*)(base + L.task_off);
+      auto pt = (const PanelTask*)(base + L.off2);
+      auto rt = (const ReflTask*)(base + L.off3);
+      auto fm = (const FlowMat*)(base + L.off4);
+#define FLOW_CASE(R, T)                                                                                     \
+  case R * 10 + T:                                                                                          \
+    hipLaunchKernelGGL((bsof_flow_kernel<R, T>), dim3(L.grid), dim3(512), 0, s, ft, L.ntasks, pt, rt, fm, \
+                       L.ctl, L.wait_ticks);                                                                \
+    break;
+      switch (L.variant) {
+        FLOW_CASE(2, 1) FLOW_CASE(4, 2) FLOW_CASE(8, 4)
+        default: break;
+      }
+#undef FLOW_CASE
+      hipLaunchKernelGGL(flow_check_kernel, dim3((L.count + 255) / 256), dim3(256), 0, s, L.ctl, L.info, L.count,
+                         BSOFI_ERR_SCHEDULE);
+      break;
+    }
     default:
       break;
   }
@@ -922,6 +1072,8 @@ static bool known_variant(const Launch& L) {
              L.variant == 116;
     case F_GEMM:
       return L.variant >= 0 && L.variant <= 2;
+    case F_FLOW:
+      return L.variant == 21 || L.variant == 42 || L.variant == 84;
     case F_SCAN:
       return scan_variant_ok(L.variant / 100, L.variant % 100);
     default:
@@ -1039,6 +1191,18 @@ int bsofi_ctx_set_lanes(bsofi_ctx_t c, int lanes) {
   return 0;
 }
 
+int bsofi_ctx_set_schedule(bsofi_ctx_t c, int mode) {
+  if (!c || (mode != 0 && mode != 1)) return -1;
+  c->schedule = mode;
+  return 0;
+}
+
+int bsofi_ctx_set_wait_limit(bsofi_ctx_t c, double seconds) {
+  if (!c || !(seconds >= 0.0) || seconds > 3600.0) return -1;
+  c->wait_s = seconds;
+  return 0;
+}
+
 int bsofi_ctx_release_cache(bsofi_ctx_t c) {
   if (!c) return -1;
   (void)hipSetDevice(c->device);
@@ -1061,14 +1225,24 @@ size_t bsofi_workspace_bytes(int n, int p) {
   return ws_layout(geometry(n, p, n * p), BSOFI_PHASE_ALL).total;
 }
 
-// Plan workspace: info words, then (sign, log|det|) per matrix, then the
-// matrices' own pieces.
+// Plan workspace: info words, then (sign, log|det|) per matrix, then (with
+// BSOF) the control region of the device-scheduled sweep (stop word and
+// queue head in the first 1 KB, then per matrix N column counters and its
+// sub-panel flags), then the matrices' own pieces.
 static size_t plan_header_bytes(int count) {
   return al256((size_t)count * sizeof(int)) + al256((size_t)count * 2 * sizeof(double));
 }
 
+static size_t flow_region_ints(int count, const bsofi_matrix_t* mats, int phases) {
+  if (!(phases & BSOFI_PHASE_BSOF) || count == 0) return 0;
+  size_t ints = 256;  // stop word at 1, queue head at 16
+  for (int i = 0; i < count; ++i)
+    ints += (size_t)mats[i].n * mats[i].p + flow_subpanels_max(mats[i].n, mats[i].p);
+  return ints;
+}
+
 size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int phases) {
-  size_t total = plan_header_bytes(count);
+  size_t total = plan_header_bytes(count) + al256(flow_region_ints(count, mats, phases) * sizeof(int));
   for (int i = 0; i < count; ++i) total += ws_layout(geometry(mats[i].n, mats[i].p, mats[i].lda), phases).total;
   return total;
 }
@@ -1104,7 +1278,10 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   P->d_det = (double*)(P->ws + al256((size_t)count * sizeof(int)));
   P->has_det = (phases & BSOFI_PHASE_BSOF) != 0;
   std::vector<MatJob> jobs(count);
-  size_t off = plan_header_bytes(count);
+  const size_t flow_ints = flow_region_ints(count, mats, phases);
+  int* flow_ctl = (int*)(P->ws + plan_header_bytes(count));
+  size_t off = plan_header_bytes(count) + al256(flow_ints * sizeof(int));
+  size_t flow_off = 256;
   for (int i = 0; i < count; ++i) {
     MatJob& J = jobs[i];
     J.g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
@@ -1121,6 +1298,12 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     J.Z2 = (double*)(b + L.Z2);
     J.Y = (double*)(b + L.Y);
     off += L.total;
+    if (flow_ints) {
+      J.fctr = flow_ctl + flow_off;
+      flow_off += (size_t)J.g.N;
+      J.fdone = flow_ctl + flow_off;
+      flow_off += flow_subpanels_max(J.g.n, J.g.p);
+    }
   }
   int max_shmem = 65536;
   (void)hipDeviceGetAttribute(&max_shmem, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
@@ -1161,13 +1344,26 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
       load[l] += work(i);
     }
   }
+  // BSOF of every matrix as one device-scheduled launch on the context's
+  // stream ahead of the lanes, where every matrix has a flow-kernel panel
+  const bool flow = (phases & BSOFI_PHASE_BSOF) && ctx->schedule == 1 && Planner::flow_ok(jobs);
+  if (flow) {
+    int rate_khz = 100000;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess ||
+        rate_khz <= 0)
+      rate_khz = 100000;
+    const long long ticks = (long long)(ctx->wait_s * 1e3 * rate_khz);
+    pl.flow_bsof(jobs, flow_ctl, (long long)flow_ints, P->d_info, count, ticks);
+    pl.det(jobs);
+    for (auto& L : P->launches) L.prefork = true;
+  }
   for (int l = 0; l < P->lanes; ++l) {
     std::vector<MatJob> grp;
     for (int i = 0; i < count; ++i)
       if (lane_of[i] == l) grp.push_back(jobs[i]);
     if (grp.empty()) continue;
     pl.lane = l;
-    if (phases & BSOFI_PHASE_BSOF) {
+    if ((phases & BSOFI_PHASE_BSOF) && !flow) {
       pl.bsof(grp);
       pl.det(grp);
     }
@@ -1227,6 +1423,25 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
 // launches are issued alternately, so that neither waits on the host for the
 // other's.  Under stream capture the same calls record a graph with two
 // branches.
+static void run_one(bsofi_plan_s* P, const Launch& L, hipStream_t s) {
+  bool timed = (L.family == P->timed_family);
+  if (timed) {
+    // a launch whose events cannot be created runs untimed
+    while ((size_t)P->ev_used + 2 > P->ev.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) break;
+      P->ev.push_back(e);
+    }
+    timed = (size_t)P->ev_used + 2 <= P->ev.size();
+    if (timed) (void)hipEventRecord(P->ev[P->ev_used], s);
+  }
+  run_launch(L, P->d_blob, s);
+  if (timed) {
+    (void)hipEventRecord(P->ev[P->ev_used + 1], s);
+    P->ev_used += 2;
+  }
+}
+
 static void run_all(bsofi_plan_s* P) {
   hipStream_t st[2] = {P->ctx->stream, P->lanes == 2 ? P->ctx->side : P->ctx->stream};
   std::vector<const Launch*> lane[2];
@@ -1234,6 +1449,10 @@ static void run_all(bsofi_plan_s* P) {
     if (L.family == F_FILL) {
       hipLaunchKernelGGL(fill_int_kernel, dim3((L.ntasks + 255) / 256), dim3(256), 0, st[0], P->d_info, L.ntasks,
                          INT_MAX);
+      continue;
+    }
+    if (L.prefork) {
+      run_one(P, L, st[0]);
       continue;
     }
     lane[L.lane & 1].push_back(&L);
@@ -1247,26 +1466,7 @@ static void run_all(bsofi_plan_s* P) {
   for (size_t i = 0; i < std::max(lane[0].size(), lane[1].size()); ++i)
     for (int l = 0; l < 2; ++l)
       if (i < lane[l].size()) order.push_back(lane[l][i]);
-  for (const Launch* Lp : order) {
-    const Launch& L = *Lp;
-    hipStream_t s = st[L.lane & 1];
-    bool timed = (L.family == P->timed_family);
-    if (timed) {
-      // a launch whose events cannot be created runs untimed
-      while ((size_t)P->ev_used + 2 > P->ev.size()) {
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) break;
-        P->ev.push_back(e);
-      }
-      timed = (size_t)P->ev_used + 2 <= P->ev.size();
-      if (timed) (void)hipEventRecord(P->ev[P->ev_used], s);
-    }
-    run_launch(L, P->d_blob, s);
-    if (timed) {
-      (void)hipEventRecord(P->ev[P->ev_used + 1], s);
-      P->ev_used += 2;
-    }
-  }
+  for (const Launch* Lp : order) run_one(P, *Lp, st[Lp->lane & 1]);
   if (P->lanes == 2) {
     (void)hipEventRecord(P->join, st[1]);
     (void)hipStreamWaitEvent(st[0], P->join, 0);

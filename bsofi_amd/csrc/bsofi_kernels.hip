@@ -320,13 +320,13 @@ __device__ __forceinline__ int wave_reduce_entry(int lane) {
   return jj;
 }
 
+// The body of one sub-panel QR by one workgroup of 64 NW threads (all of
+// them must call it); the kernels below and the device-scheduled BSOF sweep
+// run it.
 template <int RPG, int CPW, int NW>
-__global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
-                                                               const Item* __restrict__ items) {
+__device__ __forceinline__ void panel_body(const PanelTask& t) {
   constexpr int NB = NW * CPW;
   constexpr int NT = 64 * NW;
-  const Item it = items[blockIdx.x];
-  const PanelTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int m = t.m, nbj = t.nbj, lda = t.lda;
@@ -487,6 +487,12 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
   }
 }
 
+template <int RPG, int CPW, int NW>
+__global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
+                                                               const Item* __restrict__ items) {
+  panel_body<RPG, CPW, NW>(tasks[items[blockIdx.x].task]);
+}
+
 // ---------------------------------------------------------------------------
 // Block reflector application (K2)
 // ---------------------------------------------------------------------------
@@ -522,13 +528,12 @@ __device__ __forceinline__ double vload(gptr<const double> V, int ldv, int m, in
 // and is stored (a workgroup barrier waits for LDS operations only, so the
 // loads stay in flight across it).  Without MS a workgroup takes the one slab
 // it.sub.  Every slab goes through the same arithmetic either way.
+// The body for slab it.sub of task t (or slabs it.sub .. it.sub + spw - 1
+// with MS) by one workgroup of 64 NW threads.
 template <int RT, int NW, bool MS = false>
-__global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
-                                                             const Item* __restrict__ items, int spw) {
+__device__ __forceinline__ void refl_body(const ReflTask& t, const Item it, int spw) {
   static_assert(!MS || RT <= 4, "the multi-slab form holds V in registers");
   constexpr int NT = 64 * NW;
-  const Item it = items[blockIdx.x];
-  const ReflTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int m = t.m, nbj = t.nbj;
@@ -701,6 +706,141 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
       for (int r = 0; r < 4; ++r) cp[r] = cn[r];
     }
   }
+}
+
+template <int RT, int NW, bool MS = false>
+__global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
+                                                             const Item* __restrict__ items, int spw) {
+  const Item it = items[blockIdx.x];
+  refl_body<RT, NW, MS>(tasks[it.task], it, spw);
+}
+
+// ---------------------------------------------------------------------------
+// Device-scheduled BSOF sweep
+// ---------------------------------------------------------------------------
+// Number of sub-panels before g (step-major index within one matrix) whose
+// reflectors act on column c.  Column c lies in block column b:
+//   * b = p-1 (the last block column) is target 2 of every sub-panel of
+//     steps 0 .. p-3: indexes [0, s0 (p-2));
+//   * 1 <= b <= p-2 is target 1 of every sub-panel of step b-1:
+//     [s0 (b-1), s0 b);
+//   * then, inside the step kc whose panel holds c (kc = b, or p-2 for the
+//     last two block columns), every earlier sub-panel of that step:
+//     [s0 kc, s0 kc + jc), jc the sub-panel holding c.
+// (Steps 0 .. p-3 have s0 sub-panels each, so step k starts at s0 k.)
+__device__ __forceinline__ int flow_touch_before(const FlowMat& M, int c, int g) {
+  const int n = M.n, p = M.p, s0 = M.s0;
+  const int b = c / n;
+  int a1 = 0, b1 = 0;
+  if (b == p - 1) {
+    b1 = (p - 2) * s0;
+  } else if (b >= 1) {
+    a1 = (b - 1) * s0;
+    b1 = b * s0;
+  }
+  const int kc = b <= p - 3 ? b : p - 2;
+  const int a2 = kc * s0, b2 = a2 + (c - kc * n) / M.nb;
+  return max(0, min(b1, g) - a1) + max(0, min(b2, g) - a2);
+}
+
+__device__ __forceinline__ int ld_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Column of a task that lane `lane` of wave 0 watches (-1: none): the
+// sub-panel's own columns for a QR, the slab's 16 columns for a reflector
+// task.
+__device__ __forceinline__ int flow_column(const FlowTask& f, const PanelTask* pt, const ReflTask* rt, int lane) {
+  if (f.kind == 0) return lane < pt[f.ref].nbj ? f.col0 + lane : -1;
+  if (lane >= 16) return -1;
+  const ReflTask& t = rt[f.ref];
+  const int vcol = f.sub * 16 + lane;
+  if (vcol < t.w1) return f.col0 + vcol;
+  if (vcol - t.w1 < t.w2) return f.col2 + (vcol - t.w1);
+  return -1;
+}
+
+// One persistent launch of 512-thread workgroups.  A workgroup takes the
+// next task from the queue (ctl[16] is its head, ft[0 .. ntasks) the
+// tasks); wave 0 waits until every column the task touches has its counter
+// at the task's count (and, for a reflector task, until its sub-panel's QR
+// is stored), then the workgroup runs the task with the panel / reflector
+// bodies of the per-step kernels (the same arithmetic, so the same bits),
+// waits for its stores, and wave 0 publishes the new counters or the
+// sub-panel flag.  Publication is a device-scope release (the XCD's L2 is
+// written back, so a workgroup on another XCD sees the data), the wait a
+// device-scope acquire (stale L1 / L2 lines are dropped).  The queue lists
+// tasks in an order where every task's inputs come from earlier tasks, so
+// the earliest unfinished task can always run and the sweep cannot
+// deadlock, whatever number of workgroups is resident.  Every wait is
+// bounded by wait_ticks of the constant wall clock: past it the workgroup
+// sets ctl[1], every workgroup stops taking tasks and leaves, and
+// flow_check_kernel reports the error through the matrices' info words.
+template <int RPG, int RT>
+__global__ __launch_bounds__(512) void bsof_flow_kernel(const FlowTask* __restrict__ ft, int ntasks,
+                                                        const PanelTask* __restrict__ pt,
+                                                        const ReflTask* __restrict__ rt,
+                                                        const FlowMat* __restrict__ fm, int* ctl,
+                                                        long long wait_ticks) {
+  __shared__ int s_task, s_ok;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (;;) {
+    if (tid == 0) s_task = ld_agent(ctl + 1) != 0 ? ntasks : atomicAdd(ctl + 16, 1);
+    __syncthreads();
+    const int i = s_task;
+    if (i >= ntasks) break;
+    const FlowTask f = ft[i];
+    const FlowMat M = fm[f.mat];
+    if (wave == 0) {
+      const int c = flow_column(f, pt, rt, lane);
+      const int need = c >= 0 ? flow_touch_before(M, c, f.g) : 0;
+      const long long t0 = wall_clock64();
+      bool ok = true;
+      for (;;) {
+        bool ready = c < 0 || ld_agent(M.ctr + c) >= need;
+        if (f.kind == 1) ready = ready && ld_agent(M.done + f.g) != 0;
+        if (__all(ready)) break;
+        if (ld_agent(ctl + 1) != 0 || wall_clock64() - t0 > wait_ticks) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (lane == 0) {
+        if (!ok) st_agent(ctl + 1, 1);
+        s_ok = ok;
+      }
+      // what the counters published is visible to this workgroup
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (!s_ok) break;
+    if (f.kind == 0)
+      panel_body<RPG, 4, 8>(pt[f.ref]);
+    else
+      refl_body<RT, 8, false>(rt[f.ref], Item{f.ref, f.sub}, 1);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have reached L2
+    __syncthreads();
+    if (wave == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (f.kind == 0) {
+        if (lane == 0) st_agent(M.done + f.g, 1);
+      } else {
+        const int c = flow_column(f, pt, rt, lane);
+        if (c >= 0) st_agent(M.ctr + c, flow_touch_before(M, c, f.g) + 1);
+      }
+    }
+  }
+}
+
+// After a flow launch: a sweep that stopped on a wait limit marks every
+// matrix of the plan with info = code.
+__global__ void flow_check_kernel(const int* __restrict__ ctl, int* __restrict__ info, int count, int code) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count && ctl[1] != 0) info[e] = code;
 }
 
 // Reflector application for slabs taller than the register-resident forms

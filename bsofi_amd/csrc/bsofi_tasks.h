@@ -88,6 +88,35 @@ struct DetTask {
   int lda, N;
 };
 
+// Device-scheduled BSOF sweep (bsof_flow_kernel).  One persistent launch
+// runs every sub-panel QR (kind 0) and every application of a sub-panel's
+// reflectors to one 16-column slab (kind 1) of all matrices of a plan;
+// workgroups take tasks from a device queue in a dependency-respecting order
+// and wait, with a time limit, for what each task reads:
+//   * per matrix, one counter per column of A: how many sub-panels'
+//     reflectors have been applied to that column so far;
+//   * per matrix, one flag per sub-panel: its QR (V, T, tau) is stored.
+// A task on column c of sub-panel index g (step-major order of the BSOF
+// sweep) needs the counter of c to have reached the number of sub-panels
+// before g whose reflectors touch c, a closed form of (n, p, nb, c, g).
+struct FlowMat {
+  int* ctr;   // N column counters
+  int* done;  // one flag per sub-panel
+  int n, p, nb;
+  int s0;     // sub-panels per step in steps 0 .. p-3 (ceil(n / nb))
+};
+
+struct FlowTask {
+  int kind;  // 0: sub-panel QR, 1: reflector slab
+  int mat;   // FlowMat index
+  int g;     // sub-panel index within its matrix
+  int ref;   // PanelTask (kind 0) or ReflTask (kind 1) index
+  int sub;   // kind 1: 16-column slab of the ReflTask
+  int col0;  // kind 0: first column of the sub-panel; kind 1: column of C1's first column
+  int col2;  // kind 1: column of C2's first column
+  int pad;
+};
+
 // Work-item record: which task a workgroup serves and which piece of it.
 struct Item {
   int task;

@@ -35,6 +35,7 @@ extern "C" {
 #define BSOFI_ERR_ALLOC (-1001)     /* device allocation failed */
 #define BSOFI_ERR_UNSUPPORTED (-1002) /* n beyond the supported range */
 #define BSOFI_ERR_HANDLE (-1003)    /* uninitialised or finalized H-API handle */
+#define BSOFI_ERR_SCHEDULE (-1004)  /* the device-scheduled BSOF sweep hit its wait limit */
 
 /* ======================================================================
  * Part 1: reference-compatible entry points (host arrays)
@@ -186,6 +187,28 @@ int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
  * once on several contexts (the serving loop) want 1.  Returns -1 for other
  * values.  No counterpart in the reference. */
 int bsofi_ctx_set_lanes(bsofi_ctx_t ctx, int lanes);
+/* How plans created on the context afterwards run BSOF.  0 (the default):
+ * one launch per sweep step (panel, then reflectors), scheduled on the host.
+ * 1: where every matrix of the plan has 2n <= 512 and 32-wide sub-panels, the
+ * whole sweep of all matrices is ONE persistent launch that schedules itself
+ * on the device: workgroups take sub-panel QRs and reflector-slab updates
+ * from a device queue ordered so that every task's inputs come from earlier
+ * tasks, and wait on per-column counters and per-sub-panel flags for them
+ * (the next sub-panel's QR starts as soon as its own columns are updated,
+ * beside the updates of the rest).  Both give the same bits; on MI355X the
+ * device-scope release / acquire every task needs (the XCDs' L2 caches are
+ * not coherent) costs more than the launches it saves, so 1 is measured
+ * slower and is not the default.  The
+ * reference's CPU/GPU load balancing (src/bsofi_hybrid_load_balancing.c,
+ * KAPPA_* and *_SWITCH_*) is what this replaces.  Returns -1 for other
+ * values. */
+int bsofi_ctx_set_schedule(bsofi_ctx_t ctx, int mode);
+/* Wait limit of the device-scheduled sweep: a workgroup that waits longer
+ * than this for a task's inputs stops the sweep, and every matrix of the
+ * plan reports info = BSOFI_ERR_SCHEDULE instead of the GPU hanging
+ * (default 4 s; 0 makes any wait fail, for tests of that path).  Returns -1
+ * outside [0, 3600]. */
+int bsofi_ctx_set_wait_limit(bsofi_ctx_t ctx, double seconds);
 /* Destroyed plans return their device buffers (task tables, workspace) to
  * the context, and later plans on it reuse them, so that creating and
  * destroying plans in a loop allocates nothing once warm.  These report and
@@ -228,7 +251,8 @@ int bsofi_det(bsofi_ctx_t ctx, const double* dA, int lda, int n, int p, const do
 /* Time every launch of one kernel family with HIP events on the plan's
  * stream; read the accumulated milliseconds and launch count after a sync.
  * family: 0 panel QR, 1 reflector apply, 2 tile GEMM, 3 diag TRTRI,
- * 4 identity fill, 5 BSOI scan, 6 T factor. */
+ * 4 identity fill, 5 BSOI scan, 6 T factor, 8 determinant, 9 the
+ * device-scheduled BSOF sweep. */
 int bsofi_plan_time_family(bsofi_plan_t plan, int family);
 int bsofi_plan_family_time(bsofi_plan_t plan, double* ms, int* launches);
 int bsofi_plan_destroy(bsofi_plan_t plan);

@@ -482,3 +482,80 @@ def test_fp64_rate_calibration(B):
     with pytest.raises(RuntimeError):
         B.fp64_rate(ctx, 3)
     ctx.close()
+
+
+def _run_plan(B, inputs, sizes, schedule, graph=False, reps=1, wait=None, phases=None):
+    """Invert `inputs` (host arrays) in one plan on a fresh context with the
+    given BSOF schedule; returns (results, info of the last execution)."""
+    import torch
+
+    ctx = B.Context(0)
+    ctx.set_schedule(schedule)
+    if wait is not None:
+        ctx.set_wait_limit(wait)
+    devs = [to_dev(A0) for A0 in inputs]
+    taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in sizes]
+    torch.cuda.synchronize()
+    kw = {} if phases is None else {"phases": phases}
+    plan = B.Plan(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p) for d, t, (n, p) in zip(devs, taus, sizes)], **kw)
+    plan.use_graph(graph)
+    out, info = None, None
+    for r in range(reps):
+        for d, A0 in zip(devs, inputs):
+            d.copy_(to_dev(A0))
+        torch.cuda.synchronize()
+        plan.execute()
+        ctx.synchronize()
+        info = plan.info()
+        got = [from_dev(d, n * p).copy() for d, (n, p) in zip(devs, sizes)]
+        if out is not None:
+            for a, b in zip(got, out):
+                assert np.array_equal(a, b), f"replay {r} differs"
+        out = got
+    plan.close()
+    ctx.close()
+    return out, info
+
+
+@pytest.mark.parametrize("case", ["single_n256", "single_small", "mix", "graph"])
+def test_device_schedule_same_bits(B, eps, case):
+    """BSOF as one device-scheduled launch (workgroups take sub-panel QRs and
+    reflector slabs from a device queue, waiting on column counters) gives
+    the bits of the launch-per-step schedule: one n = 256 matrix, small
+    single matrices (n = 1 .. 144), the mixed batch, and graph replays."""
+    if case == "single_n256":
+        specs = [[(16, 8)]]
+    elif case == "single_small":
+        specs = [[(1, 4)], [(3, 5)], [(6, 3)], [(8, 6)], [(12, 4)]]
+    elif case == "mix":
+        specs = [[(lx, p) for lx, p in MIX]]
+    else:
+        specs = [[(4, 6), (8, 5), (16, 2)]]
+    for sp in specs:
+        inputs = [dqmc.dqmc_matrix(lx, p, instance=60 + k)[0] for k, (lx, p) in enumerate(sp)]
+        sizes = [(lx * lx, p) for lx, p in sp]
+        print(f"schedule case {case}: {sizes}", flush=True)
+        graph = case == "graph"
+        ref, info0 = _run_plan(B, inputs, sizes, 0, graph=graph, reps=3 if graph else 1)
+        got, info1 = _run_plan(B, inputs, sizes, 1, graph=graph, reps=3 if graph else 1)
+        assert info0 == [0] * len(sp) and info1 == [0] * len(sp)
+        for k, (a, b) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, b), f"matrix {k} {sizes[k]}: device schedule differs"
+            assert oracle.residual(inputs[k], a) <= 100 * sizes[k][0] * sizes[k][1] * eps
+
+
+def test_device_schedule_wait_limit_reports_error(B):
+    """A wait limit of zero makes the first unmet dependency stop the
+    device-scheduled sweep: every workgroup leaves, every matrix of the plan
+    reports BSOFI_ERR_SCHEDULE, and the context runs the next plan normally
+    with the limit restored."""
+    sp = [(16, 4), (8, 6)]
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=80 + k)[0] for k, (lx, p) in enumerate(sp)]
+    sizes = [(lx * lx, p) for lx, p in sp]
+    _, info = _run_plan(B, inputs, sizes, 1, wait=0.0, phases=B.PHASE_BSOF)
+    assert info == [B.BSOFI_ERR_SCHEDULE] * len(sp)
+    ref, info0 = _run_plan(B, inputs, sizes, 0)
+    got, info1 = _run_plan(B, inputs, sizes, 1, wait=4.0)
+    assert info0 == [0, 0] and info1 == [0, 0]
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
