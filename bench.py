@@ -546,7 +546,8 @@ def main():
     if args.e2e:
         e = bench_e2e(B, args.lx, args.p, args.seed)
         line["e2e_host_api"] = {
-            "path": "hybridxbsoftrih_ + hybridxbsoih_ on host arrays (A in, X out, G out over PCIe)",
+            "path": "hybridxinith_ (outside the timing), then hybridxbsoftrih_ + hybridxbsoih_ on host arrays: the "
+                    "2p blocks of A in, V + X out, V + X in again, G out, through pinned double-buffered staging",
             "seconds": round(e["seconds"], 3),
             "inversions_per_s": round(1.0 / e["seconds"], 3),
             "info": list(e["info"]),

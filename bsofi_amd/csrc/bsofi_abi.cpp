@@ -4,18 +4,25 @@
 // info convention of its SGo-Go/BSOFI counterpart (cited per function) and
 // runs the computation on the GPU through the native API of
 // bsofi_engine.hip.  Host arrays are staged through device buffers that
-// the library keeps between calls; nothing is computed on the CPU.
+// the library keeps between calls, via pinned host buffers and a few copy
+// threads; nothing is computed on the CPU.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <sys/time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "bsofi.h"
 #include "bsofi_internal.h"
@@ -23,8 +30,100 @@
 namespace {
 
 // ---------------------------------------------------------------------------
+// Parallel copies between pinned staging buffers and the caller's arrays
+// ---------------------------------------------------------------------------
+// A small pool of host threads: run(count, fn) calls fn(i) for i < count,
+// spread over the workers and the calling thread, and returns when all are
+// done.  The caller's arrays are ordinary pageable memory (registering them
+// is not available), so host-array transfers go through pinned buffers; the
+// host-side copy into or out of them is the part threads speed up.
+class CopyPool {
+ public:
+  explicit CopyPool(int workers) {
+    for (int t = 0; t < workers; ++t) th_.emplace_back([this] { worker(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void run(int count, const std::function<void(int)>& fn) {
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [&] { return active_ == 0; });  // no worker still inside the previous job
+      fn_ = &fn;
+      count_ = count;
+      next_.store(0);
+      ++gen_;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return active_ == 0; });
+  }
+
+ private:
+  void drain() {
+    for (int i = next_.fetch_add(1); i < count_; i = next_.fetch_add(1)) (*fn_)(i);
+  }
+  void worker() {
+    long seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      ++active_;
+      lk.unlock();
+      drain();
+      lk.lock();
+      if (--active_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int count_ = 0;
+  std::atomic<int> next_{0};
+  int active_ = 0;
+  long gen_ = 0;
+  bool stop_ = false;
+};
+
+// A column-major rectangle of A: columns [c0, c1), rows [r0, r1).
+struct Rect {
+  int c0, c1, r0, r1;
+};
+
+// The non-zero blocks of a p-cyclic input: block column k rows k and k+1
+// (the diagonal and subdiagonal blocks), and the corner block A_BLK(0, L-1).
+// The rest of the device copy is zero-filled: the reference's calls read
+// those blocks as the zeros the p-cyclic contract puts there.
+void rects_input(int n, int L, std::vector<Rect>& out) {
+  const int N = n * L;
+  for (int k = 0; k < L; ++k) out.push_back({k * n, (k + 1) * n, k * n, std::min((k + 2) * n, N)});
+  out.push_back({(L - 1) * n, L * n, 0, n});
+}
+
+// What BSOF and BSTRI leave in A (V + R, V + X): block column k holds data
+// in rows [0, (k+2) n) only -- R or X above, the reflectors in its diagonal
+// and subdiagonal blocks -- and the caller's zeros below.
+void rects_factor(int n, int L, std::vector<Rect>& out) {
+  const int N = n * L;
+  for (int k = 0; k < L; ++k) out.push_back({k * n, (k + 1) * n, 0, std::min((k + 2) * n, N)});
+}
+
+void rects_full(int N, std::vector<Rect>& out) { out.push_back({0, N, 0, N}); }
+
+// ---------------------------------------------------------------------------
 // Device staging for host-array calls
 // ---------------------------------------------------------------------------
+constexpr size_t kSlotBytes = (size_t)64 << 20;  // two pinned slots of 64 MiB
+
 struct Staging {
   bsofi_ctx_t ctx = nullptr;
   double* dA = nullptr;
@@ -33,6 +132,9 @@ struct Staging {
   size_t capTau = 0;
   char* ws = nullptr;
   size_t capWs = 0;
+  char* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::unique_ptr<CopyPool> pool;
   int failed = 0;
 
   int ensure_ctx() {
@@ -45,6 +147,29 @@ struct Staging {
       if (!once++) fprintf(stderr, "bsofi_amd: no usable GPU (code %d); BSOF/I calls return info=%d\n", rc, rc);
     }
     return rc;
+  }
+  // pinned slots, their events and the copy threads, on first use
+  int ensure_pinned() {
+    if (pin[0]) return 0;
+    for (int s = 0; s < 2; ++s) {
+      if (hipHostMalloc((void**)&pin[s], kSlotBytes, hipHostMallocDefault) != hipSuccess ||
+          hipEventCreateWithFlags(&ev[s], hipEventDisableTiming) != hipSuccess) {
+        release_pinned();
+        return BSOFI_ERR_ALLOC;
+      }
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    pool.reset(new CopyPool((int)std::min(15u, hw - 1)));
+    return 0;
+  }
+  void release_pinned() {
+    for (int s = 0; s < 2; ++s) {
+      if (pin[s]) (void)hipHostFree(pin[s]);
+      if (ev[s]) (void)hipEventDestroy(ev[s]);
+      pin[s] = nullptr;
+      ev[s] = nullptr;
+    }
+    pool.reset();
   }
   static int grow(void** p, size_t* cap, size_t need) {
     if (*cap >= need && *p) return 0;
@@ -59,12 +184,14 @@ struct Staging {
     return 0;
   }
   void release() {
+    if (ctx) (void)bsofi_ctx_synchronize(ctx);
     if (dA) (void)hipFree(dA);
     if (dtau) (void)hipFree(dtau);
     if (ws) (void)hipFree(ws);
     dA = dtau = nullptr;
     ws = nullptr;
     capA = capTau = capWs = 0;
+    release_pinned();
     if (ctx) bsofi_ctx_destroy(ctx);
     ctx = nullptr;
   }
@@ -75,27 +202,100 @@ Staging g_stage;
 
 inline int ld_dev(int N) { return ((N + 15) / 16) * 16; }
 
-int upload(bsofi_ctx_t ctx, double* dA, int ldda, const double* A, int lda, int N) {
-  hipStream_t s = (hipStream_t)bsofi_ctx_stream(ctx);
-  if (hipMemcpy2DAsync(dA, (size_t)ldda * 8, A, (size_t)lda * 8, (size_t)N * 8, N, hipMemcpyHostToDevice, s) !=
-      hipSuccess)
-    return BSOFI_ERR_NO_DEVICE;
+// Pieces of at most one slot: consecutive columns of one rectangle.
+void chunks(const std::vector<Rect>& rects, std::vector<Rect>& out) {
+  for (const Rect& r : rects) {
+    const size_t col_bytes = (size_t)(r.r1 - r.r0) * 8;
+    if (col_bytes == 0) continue;
+    const int per = (int)std::max<size_t>(1, kSlotBytes / col_bytes);
+    for (int c = r.c0; c < r.c1; c += per) out.push_back({c, std::min(r.c1, c + per), r.r0, r.r1});
+  }
+}
+
+// Host -> device for the rectangles: the pool packs chunk i into a pinned
+// slot while the DMA of chunk i - 1 runs from the other.
+int upload(Staging& st, const std::vector<Rect>& rects, const double* A, int lda, double* dA, int ldda) {
+  hipStream_t s = (hipStream_t)bsofi_ctx_stream(st.ctx);
+  std::vector<Rect> ch;
+  chunks(rects, ch);
+  for (size_t i = 0; i < ch.size(); ++i) {
+    const Rect& r = ch[i];
+    const int slot = (int)(i & 1);
+    const size_t rows = (size_t)(r.r1 - r.r0);
+    if (rows * 8 > kSlotBytes) {  // a column larger than a slot: direct from pageable memory
+      if (hipMemcpy2DAsync(dA + (size_t)r.c0 * ldda + r.r0, (size_t)ldda * 8, A + (size_t)r.c0 * lda + r.r0,
+                           (size_t)lda * 8, rows * 8, r.c1 - r.c0, hipMemcpyHostToDevice, s) != hipSuccess)
+        return BSOFI_ERR_NO_DEVICE;
+      continue;
+    }
+    if (i >= 2 && hipEventSynchronize(st.ev[slot]) != hipSuccess) return BSOFI_ERR_NO_DEVICE;
+    double* buf = (double*)st.pin[slot];
+    st.pool->run(r.c1 - r.c0, [&](int j) {
+      memcpy(buf + (size_t)j * rows, A + (size_t)(r.c0 + j) * lda + r.r0, rows * 8);
+    });
+    if (hipMemcpy2DAsync(dA + (size_t)r.c0 * ldda + r.r0, (size_t)ldda * 8, buf, rows * 8, rows * 8, r.c1 - r.c0,
+                         hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipEventRecord(st.ev[slot], s) != hipSuccess)
+      return BSOFI_ERR_NO_DEVICE;
+  }
   return 0;
 }
 
-int download(bsofi_ctx_t ctx, double* A, int lda, const double* dA, int ldda, int N) {
-  hipStream_t s = (hipStream_t)bsofi_ctx_stream(ctx);
-  if (hipMemcpy2DAsync(A, (size_t)lda * 8, dA, (size_t)ldda * 8, (size_t)N * 8, N, hipMemcpyDeviceToHost, s) !=
-      hipSuccess)
-    return BSOFI_ERR_NO_DEVICE;
-  return bsofi_ctx_synchronize(ctx);
+// Device -> host: the DMA of chunk i into one slot runs while the pool
+// unpacks chunk i - 1 from the other into the caller's array.
+int download(Staging& st, const std::vector<Rect>& rects, double* A, int lda, const double* dA, int ldda) {
+  hipStream_t s = (hipStream_t)bsofi_ctx_stream(st.ctx);
+  std::vector<Rect> ch;
+  chunks(rects, ch);
+  auto unpack = [&](size_t i) -> int {
+    const Rect& r = ch[i];
+    const size_t rows = (size_t)(r.r1 - r.r0);
+    if (rows * 8 > kSlotBytes) return 0;
+    const int slot = (int)(i & 1);
+    if (hipEventSynchronize(st.ev[slot]) != hipSuccess) return BSOFI_ERR_NO_DEVICE;
+    const double* buf = (const double*)st.pin[slot];
+    st.pool->run(r.c1 - r.c0, [&](int j) {
+      memcpy(A + (size_t)(r.c0 + j) * lda + r.r0, buf + (size_t)j * rows, rows * 8);
+    });
+    return 0;
+  };
+  for (size_t i = 0; i < ch.size(); ++i) {
+    const Rect& r = ch[i];
+    const size_t rows = (size_t)(r.r1 - r.r0);
+    if (rows * 8 > kSlotBytes) {
+      if (hipMemcpy2DAsync(A + (size_t)r.c0 * lda + r.r0, (size_t)lda * 8, dA + (size_t)r.c0 * ldda + r.r0,
+                           (size_t)ldda * 8, rows * 8, r.c1 - r.c0, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return BSOFI_ERR_NO_DEVICE;
+    } else {
+      const int slot = (int)(i & 1);
+      // the slot's previous chunk (i - 2) was unpacked at step i - 1
+      if (hipMemcpy2DAsync(st.pin[slot], rows * 8, dA + (size_t)r.c0 * ldda + r.r0, (size_t)ldda * 8, rows * 8,
+                           r.c1 - r.c0, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipEventRecord(st.ev[slot], s) != hipSuccess)
+        return BSOFI_ERR_NO_DEVICE;
+    }
+    if (i >= 1) {
+      int rc = unpack(i - 1);
+      if (rc) return rc;
+    }
+  }
+  if (!ch.empty()) {
+    int rc = unpack(ch.size() - 1);
+    if (rc) return rc;
+  }
+  return bsofi_ctx_synchronize(st.ctx);
 }
 
-// Upload A (and tau), run the phases, download A (and tau).  dwork, when
-// large enough, supplies all device memory (A copy, tau, workspace).
+// Upload A (and tau), run the phases, download A (and tau).  Only what the
+// phases read and write moves: a call that starts with BSOF reads the
+// p-cyclic blocks, one that starts later the V + R / V + X region; a call
+// that ends with BSOI writes the dense G, one that ends earlier the
+// V + R / V + X region.  dwork, when large enough, supplies all device
+// memory (A copy, tau, workspace).
 int run_host(Staging& st, int n, int L, double* A, int lda, double* tau, int phases, bool tau_in, bool tau_out,
              double* dwork, long long ldwork, int* info) {
   int rc = st.ensure_ctx();
+  if (!rc) rc = st.ensure_pinned();
   if (rc) {
     *info = rc;
     return rc;
@@ -113,8 +313,9 @@ int run_host(Staging& st, int n, int L, double* A, int lda, double* tau, int pha
   double* dA;
   double* dtau;
   char* ws;
-  if (dwork && ldwork > 0 && (size_t)ldwork * 8 >= bytesA + bytesT + wsb + 256) {
-    char* base = (char*)(((uintptr_t)dwork + 255) & ~(uintptr_t)255);
+  const uintptr_t base = dwork ? (((uintptr_t)dwork + 255) & ~(uintptr_t)255) : 0;
+  const size_t need = dwork ? (size_t)(base - (uintptr_t)dwork) + ((bytesA + 255) & ~(size_t)255) + bytesT + wsb : 0;
+  if (dwork && ldwork > 0 && (size_t)ldwork * 8 >= need) {
     dA = (double*)base;
     dtau = (double*)(base + ((bytesA + 255) & ~(size_t)255));
     ws = (char*)dtau + bytesT;
@@ -129,11 +330,25 @@ int run_host(Staging& st, int n, int L, double* A, int lda, double* tau, int pha
     dtau = st.dtau;
     ws = st.ws;
   }
-  if ((rc = upload(st.ctx, dA, ldda, A, lda, N))) {
+  hipStream_t s = (hipStream_t)bsofi_ctx_stream(st.ctx);
+  std::vector<Rect> in, out;
+  if (phases & BSOFI_PHASE_BSOF) {
+    if (hipMemset2DAsync(dA, (size_t)ldda * 8, 0, (size_t)N * 8, N, s) != hipSuccess) {
+      *info = BSOFI_ERR_NO_DEVICE;
+      return *info;
+    }
+    rects_input(n, L, in);
+  } else {
+    rects_factor(n, L, in);
+  }
+  if (phases & BSOFI_PHASE_BSOI)
+    rects_full(N, out);
+  else
+    rects_factor(n, L, out);
+  if ((rc = upload(st, in, A, lda, dA, ldda))) {
     *info = rc;
     return rc;
   }
-  hipStream_t s = (hipStream_t)bsofi_ctx_stream(st.ctx);
   if (tau_in && hipMemcpyAsync(dtau, tau, (size_t)N * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
     *info = BSOFI_ERR_NO_DEVICE;
     return *info;
@@ -150,7 +365,7 @@ int run_host(Staging& st, int n, int L, double* A, int lda, double* tau, int pha
     *info = rc;
     return rc;
   }
-  if ((rc = download(st.ctx, A, lda, dA, ldda, N))) {
+  if ((rc = download(st, out, A, lda, dA, ldda))) {
     *info = rc;
     return rc;
   }
@@ -208,9 +423,6 @@ const int kMagic = 0x42534f46;  // "BSOF"
 
 struct HState {
   Staging st;
-  const double* cachedA = nullptr;
-  int cn = 0, cL = 0, clda = 0;
-  bool valid = false;
 };
 
 struct HHandle {
@@ -382,7 +594,8 @@ int hybridxinith_(int* handle, int* n, int* L, int* lda) {
   if (!handle || !n || !L || !lda) return -1;
   if (check_host(*n, *L, (const double*)1, *lda, nullptr, false)) return -1;
   HState* s = new HState();
-  if (s->st.ensure_ctx()) {
+  if (s->st.ensure_ctx() || s->st.ensure_pinned()) {
+    s->st.release();
     delete s;
     return -1;
   }
@@ -420,16 +633,7 @@ int hybridxbsoftrih_(int* handle, int* n, int* L, double A[], int* lda, double t
   int a = check_host(*n, *L, A, *lda, tau, true);
   if (!a && (!work || *lwork < 1)) a = -8;
   if (a) return *info = a;
-  s->valid = false;
-  int rc = run_host(s->st, *n, *L, A, *lda, tau, BSOFI_PHASE_BSOF | BSOFI_PHASE_BSTRI, false, true, nullptr, 0, info);
-  if (rc == 0) {
-    s->cachedA = A;
-    s->cn = *n;
-    s->cL = *L;
-    s->clda = *lda;
-    s->valid = true;
-  }
-  return rc;
+  return run_host(s->st, *n, *L, A, *lda, tau, BSOFI_PHASE_BSOF | BSOFI_PHASE_BSTRI, false, true, nullptr, 0, info);
 }
 
 int hybridxbsoih_(int* handle, int* n, int* L, double A[], int* lda, double tau[], double work[], int* lwork,
@@ -440,28 +644,9 @@ int hybridxbsoih_(int* handle, int* n, int* L, double A[], int* lda, double tau[
   int a = check_host(*n, *L, A, *lda, tau, true);
   if (!a && (!work || *lwork < 1)) a = -8;
   if (a) return *info = a;
-  const char* env = getenv("BSOFI_HAPI_REUPLOAD");
-  const bool force = env && env[0] == '1';
-  const bool reuse = !force && s->valid && s->cachedA == A && s->cn == *n && s->cL == *L && s->clda == *lda;
-  s->valid = false;
-  if (!reuse) return run_host(s->st, *n, *L, A, *lda, tau, BSOFI_PHASE_BSOI, true, false, nullptr, 0, info);
-  // device copy of V + R^{-1} and tau from bsoftriH is still in place
-  const int N = (*n) * (*L), ldda = ld_dev(N);
-  bsofi_matrix_t m{};
-  m.n = *n;
-  m.p = *L;
-  m.lda = ldda;
-  m.A = s->st.dA;
-  m.tau = s->st.dtau;
-  bsofi_plan_t plan = nullptr;
-  int rc = bsofi_plan_create(s->st.ctx, 1, &m, BSOFI_PHASE_BSOI, s->st.ws, &plan);
-  if (rc == 0) rc = bsofi_plan_execute(plan);
-  int inf = 0;
-  if (rc == 0) rc = bsofi_plan_info(plan, &inf);
-  bsofi_plan_destroy(plan);
-  if (rc == 0) rc = download(s->st.ctx, A, *lda, s->st.dA, ldda, N);
-  *info = rc ? rc : inf;
-  return *info;
+  // A and tau are read from the caller's arrays, as the reference's
+  // cb_hybridXbsoiH reads them: a caller may change them between the calls
+  return run_host(s->st, *n, *L, A, *lda, tau, BSOFI_PHASE_BSOI, true, false, nullptr, 0, info);
 }
 
 int hybridxfinalizeh_(int* handle) {

@@ -95,10 +95,13 @@ int hybridxbsoi_(int* n, int* L, double A[], int* lda, double tau[], double work
  * -- the simplified handle API.  The handle lives in the caller's integer
  * array (at least 8 Fortran integers, as the reference's 32-byte struct):
  * {magic, n, L, lda, context pointer}, 24 bytes.  init allocates a device
- * context with its own stream and workspace; bsoftriH uploads A, runs BSOF
- * and BSTRI, writes A = V + R^{-1} and tau back; bsoiH reuses the device
- * copy when (A, n, L, lda) match the preceding bsoftriH on this handle (set
- * BSOFI_HAPI_REUPLOAD=1 to always re-upload) and writes A = G back. */
+ * context with its own stream, workspace and pinned staging buffers;
+ * bsoftriH uploads the 2L non-zero blocks of the p-cyclic A (the rest of the
+ * device copy is zero, as the contract has it), runs BSOF and BSTRI, and
+ * writes V + R^{-1} (block column k: rows below (k+2) n untouched, they stay
+ * the caller's zeros) and tau back; bsoiH reads A (V + R^{-1}) and tau from
+ * the caller's arrays again, as the reference does, and writes A = G back.
+ * finalize on a finalized (zeroed) handle returns BSOFI_ERR_HANDLE. */
 int hybridxinith_(int* handle, int* n, int* L, int* lda);
 int hybridxbsofilworkh_(int* handle, int* n, int* L, int* lda);
 int hybridxbsoftrih_(int* handle, int* n, int* L, double A[], int* lda, double tau[],

@@ -187,24 +187,61 @@ def test_singular_R_reports_info(B):
     assert info == 3
 
 
-def test_handle_reuses_device_copy_and_reupload_env(B, O, monkeypatch):
+def test_handle_bsoi_reads_callers_arrays(B, O):
+    """bsoiH reads A and tau from the caller's arrays, as the reference's
+    cb_hybridXbsoiH does: a caller that replaces them between bsoftriH and
+    bsoiH (here by another matrix's V + X and tau from the oracle) gets the
+    inverse of that other matrix."""
     lx, p = 4, 5
     A0, m = dqmc.dqmc_matrix(lx, p)
+    A1, _ = dqmc.dqmc_matrix(lx, p, instance=9)
     n = m["n"]
     N = n * p
-    G = A0.copy(order="F")
-    O.invert(n, p, G)
-    for env in ("0", "1"):
-        monkeypatch.setenv("BSOFI_HAPI_REUPLOAD", env)
-        A = A0.copy(order="F")
-        tau = np.zeros(N)
-        h = B.HandleAPI(n, p, N, nints=8)
-        assert h.rc == 0
-        w = np.zeros(16)
-        assert h.bsoftri(A, tau, w) == 0
-        assert h.bsoi(A, tau, w) == 0
-        assert h.finalize() == 0
-        assert rel(A, G) <= 1e-10
+    X1 = A1.copy(order="F")
+    tau1 = np.zeros(N)
+    assert O.bsof(n, p, X1, tau1) == 0
+    assert O.bstri(n, p, X1) == 0
+    G1 = A1.copy(order="F")
+    O.invert(n, p, G1)
+    A = A0.copy(order="F")
+    tau = np.zeros(N)
+    h = B.HandleAPI(n, p, N, nints=8)
+    assert h.rc == 0
+    w = np.zeros(16)
+    assert h.bsoftri(A, tau, w) == 0
+    A[:] = X1
+    tau[:] = tau1
+    assert h.bsoi(A, tau, w) == 0
+    assert h.finalize() == 0
+    assert h.finalize() == B.BSOFI_ERR_HANDLE  # a finalized handle is refused, not freed twice
+    assert rel(A, G1) <= 1e-10
+
+
+@pytest.mark.parametrize("lx,p", [(6, 7), (16, 6)])
+def test_handle_api_bits_equal_device_path(B, lx, p):
+    """The host-array handle API (pinned staging; only the p-cyclic blocks go
+    up, V + X and then G come down) gives the bits of the same inversion on
+    a device-resident copy."""
+    import torch
+
+    A0, m = dqmc.dqmc_matrix(lx, p, instance=3)
+    n = m["n"]
+    N = n * p
+    A = A0.copy(order="F")
+    tau = np.zeros(N)
+    h = B.HandleAPI(n, p, N)
+    work = np.zeros(max(h.lwork(), 1))
+    assert h.bsoftri(A, tau, work) == 0
+    assert h.bsoi(A, tau, work) == 0
+    h.finalize()
+    ctx = B.Context(0)
+    d = torch.from_numpy(np.ascontiguousarray(A0.T)).cuda()
+    t = torch.zeros(N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, N)]) == [0]
+    G = d.cpu().numpy().T
+    ctx.close()
+    assert np.array_equal(A, G)
 
 
 def test_hybrid_with_caller_device_workspace(B, O):
