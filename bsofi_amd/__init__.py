@@ -36,6 +36,8 @@ BSOFI_ERR_ALLOC = -1001
 BSOFI_ERR_UNSUPPORTED = -1002
 BSOFI_ERR_HANDLE = -1003
 BSOFI_ERR_SCHEDULE = -1004
+BSOFI_ERR_SCRATCH = -1005
+NOALLOC_SCRATCH_BYTES = 128 << 20
 BSOFI_MAX_N = 1024
 
 PHASE_BSOF, PHASE_BSTRI, PHASE_BSOI, PHASE_ALL, PHASE_DIAG_ONLY = 1, 2, 4, 7, 8
@@ -109,6 +111,7 @@ def _load():
         "bsofi_plan_family_time": (i, [_vp, _dp, _ip]),
         "bsofi_plan_destroy": (i, [_vp]),
         "bsofi_invert_batched": (i, [_vp, i, C.POINTER(bsofi_matrix_t), i, _ip]),
+        "bsofi_invert_batched_noalloc": (i, [_vp, i, C.POINTER(bsofi_matrix_t), i, _ip]),
         "bsofi_gather_blocks": (i, [_vp, _vp, i, i, i, _vp]),
         "bsofi_pcyclic_residual": (i, [_vp, _vp, _vp, i, i, i, _dp]),
         "bsofi_flops": (None, [i, i, _dp]),

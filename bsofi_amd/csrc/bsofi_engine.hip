@@ -271,6 +271,7 @@ struct bsofi_plan_s {
   Blob blob;
   char* d_blob = nullptr;
   size_t blob_bytes = 0;
+  bool own_blob = true;  // false: the blob lives in caller-provided memory
   char* ws = nullptr;
   bool own_ws = false;
   size_t ws_bytes = 0;
@@ -1247,8 +1248,14 @@ size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int pha
   return total;
 }
 
-int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, void* workspace,
-                      bsofi_plan_t* out) {
+}  // extern "C"
+
+// Plan creation.  blob_dev, when given, is device memory of blob_cap bytes
+// for the task tables (else they come from the context's buffer cache); a
+// plan whose tables do not fit returns BSOFI_ERR_SCRATCH.  one_lane keeps
+// every launch on the context's stream (no side stream or events).
+static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, void* workspace,
+                       char* blob_dev, size_t blob_cap, bool one_lane, bsofi_plan_t* out) {
   if (!ctx || !out || count < 0 || (count > 0 && !mats)) return -1;
   *out = nullptr;
   for (int i = 0; i < count; ++i) {
@@ -1327,7 +1334,7 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   // wide launches) run beside the other lane's work.  A matrix's launches
   // stay in one lane and in order, and its kernels and tasks are the ones it
   // gets in any other grouping, so its result is the same bits.
-  P->lanes = count >= 2 ? ctx->lanes : 1;
+  P->lanes = (count >= 2 && !one_lane) ? ctx->lanes : 1;
   std::vector<int> lane_of(count, 0);
   if (P->lanes == 2) {
     auto work = [&](int i) {
@@ -1397,7 +1404,19 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
     }
   }
   P->blob_bytes = std::max<size_t>(P->blob.h.size(), 256);
-  P->d_blob = ctx_alloc(ctx, P->blob_bytes);
+  if (blob_dev) {
+    if (P->blob_bytes > blob_cap) {
+      if (P->fork) (void)hipEventDestroy(P->fork);
+      if (P->join) (void)hipEventDestroy(P->join);
+      if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
+      delete P;
+      return BSOFI_ERR_SCRATCH;
+    }
+    P->d_blob = blob_dev;
+    P->own_blob = false;
+  } else {
+    P->d_blob = ctx_alloc(ctx, P->blob_bytes);
+  }
   if (!P->d_blob) {
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
@@ -1406,7 +1425,7 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   if (!P->blob.h.empty() &&
       !hip_ok(hipMemcpyAsync(P->d_blob, P->blob.h.data(), P->blob.h.size(), hipMemcpyHostToDevice, ctx->stream),
               "hipMemcpyAsync(plan)")) {
-    ctx_free(ctx, P->d_blob, P->blob_bytes);
+    if (P->own_blob) ctx_free(ctx, P->d_blob, P->blob_bytes);
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
     return BSOFI_ERR_NO_DEVICE;
@@ -1414,6 +1433,13 @@ int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, in
   (void)hipStreamSynchronize(ctx->stream);
   *out = P;
   return 0;
+}
+
+extern "C" {
+
+int bsofi_plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, void* workspace,
+                      bsofi_plan_t* out) {
+  return plan_create(ctx, count, mats, phases, workspace, nullptr, 0, false, out);
 }
 
 // Issues the plan's launches.  A two-lane plan forks the side stream off the
@@ -1584,7 +1610,7 @@ int bsofi_plan_destroy(bsofi_plan_t P) {
   if (P->join) (void)hipEventDestroy(P->join);
   for (auto e : P->ev) (void)hipEventDestroy(e);
   // the stream has drained: the buffers go back to the context for reuse
-  ctx_free(P->ctx, P->d_blob, P->blob_bytes);
+  if (P->own_blob) ctx_free(P->ctx, P->d_blob, P->blob_bytes);
   if (P->own_ws) ctx_free(P->ctx, P->ws, P->ws_block);
   delete P;
   return 0;
@@ -1593,6 +1619,61 @@ int bsofi_plan_destroy(bsofi_plan_t P) {
 int bsofi_invert_batched(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, int* info) {
   bsofi_plan_t P = nullptr;
   int rc = bsofi_plan_create(ctx, count, mats, phases, nullptr, &P);
+  if (rc) return rc;
+  rc = bsofi_plan_execute(P);
+  if (rc == 0 && info) rc = bsofi_plan_info(P, info);
+  bsofi_plan_destroy(P);
+  return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The no-allocation entry point's scratch: one file-scope device array, of
+// which every GPU the library runs on holds its own copy (at its own
+// address) until the process ends.  Host code never passes `scratch` itself
+// to a kernel or a copy: that name is only a host-side handle.  The device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current, and is kept per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[BSOFI_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, char*> g_scratch_ptr;
+
+// Device address of `scratch` on the context's device (which is current).
+static char* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  char* ptr = nullptr;
+  if (!hip_ok(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !ptr)
+    return nullptr;
+  g_scratch_ptr[device] = ptr;
+  return ptr;
+}
+
+extern "C" {
+
+int bsofi_invert_batched_noalloc(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, int* info) {
+  if (!ctx || count < 0 || (count > 0 && !mats)) return -1;
+  for (int i = 0; i < count; ++i) {
+    const int a = bsofi_check_args(mats[i].n, mats[i].p, mats[i].lda);
+    if (a) return a;
+  }
+  const size_t wsb = al256(bsofi_plan_workspace_bytes(count, mats, phases));
+  if (wsb >= (size_t)BSOFI_NOALLOC_SCRATCH_BYTES) {
+    fprintf(stderr, "bsofi_amd: bsofi_invert_batched_noalloc needs %zu bytes of workspace, more than its %zu-byte "
+                    "scratch; use bsofi_invert_batched\n", wsb, (size_t)BSOFI_NOALLOC_SCRATCH_BYTES);
+    return BSOFI_ERR_SCRATCH;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  char* base = scratch_address(ctx->device);
+  if (!base) return BSOFI_ERR_NO_DEVICE;
+  bsofi_plan_t P = nullptr;
+  // workspace at the start of the scratch, the task tables behind it
+  int rc = plan_create(ctx, count, mats, phases, base, base + wsb, (size_t)BSOFI_NOALLOC_SCRATCH_BYTES - wsb, true,
+                       &P);
   if (rc) return rc;
   rc = bsofi_plan_execute(P);
   if (rc == 0 && info) rc = bsofi_plan_info(P, info);

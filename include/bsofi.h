@@ -36,6 +36,7 @@ extern "C" {
 #define BSOFI_ERR_UNSUPPORTED (-1002) /* n beyond the supported range */
 #define BSOFI_ERR_HANDLE (-1003)    /* uninitialised or finalized H-API handle */
 #define BSOFI_ERR_SCHEDULE (-1004)  /* the device-scheduled BSOF sweep hit its wait limit */
+#define BSOFI_ERR_SCRATCH (-1005)   /* the call needs more than the no-allocation scratch holds */
 
 /* ======================================================================
  * Part 1: reference-compatible entry points (host arrays)
@@ -263,6 +264,23 @@ int bsofi_plan_destroy(bsofi_plan_t plan);
 /* One-shot batched inversion on device (builds, runs and frees a plan). */
 int bsofi_invert_batched(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases,
                          int* info);
+/* bsofi_invert_batched without any device allocation, for callers that
+ * cannot allocate (HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The plan's workspace
+ * and task tables live in one fixed device array of
+ * BSOFI_NOALLOC_SCRATCH_BYTES per GPU (held from when the library is loaded
+ * until the process ends); a call whose workspace
+ * (bsofi_plan_workspace_bytes, rounded up to 256) plus task tables exceed it
+ * returns BSOFI_ERR_SCRATCH, and bsofi_invert_batched remains the way to run
+ * it.  Calls no hipMalloc / hipFree and creates no stream or event (every
+ * launch on the context's stream); the planner's host-side tables use the
+ * host heap.  Synchronous, like bsofi_invert_batched, and the same bits.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the
+ * caller. */
+#define BSOFI_NOALLOC_SCRATCH_BYTES (128u << 20)
+int bsofi_invert_batched_noalloc(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases,
+                                 int* info);
 
 /* Relative residual ||A G - I||_F / (||A||_F ||G||_F) computed on the
  * device from the 2p non-zero blocks of the original A (blocks in the
