@@ -135,7 +135,7 @@ def scan_kernel_name(n, scan_mti=0):
     return "bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form == 9 else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)"
 
 
-def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None):
+def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, families=False):
     import torch
 
     phases = B.PHASE_ALL if phases is None else phases
@@ -178,10 +178,25 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None):
     # the residual needs the whole G; diagonal-only runs are checked by the
     # tests (bit-equal to the full inverse's diagonal blocks)
     res = [ds.residual(k) for k in range(steps)] if phases == B.PHASE_ALL else [0.0]
+    # every kernel family timed on its own (HIP events around each of its
+    # launches on the library's stream), one extra inversion each, outside
+    # the timed region
+    fam = {}
+    if families:
+        spare = plans[steps]
+        for name in ("panel", "refl", "gemm", "trtri", "larft", "eye", "det", "flow", "scan"):
+            ds.rebuild([steps])
+            spare.time_family(name)
+            spare.execute()
+            ctx.synchronize()
+            ms, cnt = spare.family_time()
+            if cnt:
+                fam[name] = (ms, cnt)
+        spare.time_family(-1)
     for pl in plans:
         pl.close()
     out = dict(n=ds.n, p=p, N=ds.N, wall=wall, lat=lat, scan_ms=scan_ms / max(scan_n, 1), scan_launches=scan_n,
-               infos=infos, resid=res)
+               infos=infos, resid=res, families=fam, nb=ctx.get_tuning(ds.n)[0])
     del ds
     ctx.close()
     torch.cuda.empty_cache()
@@ -347,42 +362,126 @@ def bench_e2e(B, lx, p, seed):
     return dict(seconds=dt, info=(i1, i2))
 
 
-def cpu_model():
+def host_info():
+    """The host the CPU baseline runs on: the cores this process may use
+    (affinity), the cgroup CPU quota, physical cores among them, and the CPU
+    model string as /proc/cpuinfo reports it."""
+    aff = sorted(os.sched_getaffinity(0))
+    quota = None
     try:
-        with open("/proc/cpuinfo") as f:
-            for ln in f:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        quota = None if q == "max" else float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model, phys = "unknown", set()
+    try:
+        for blk in open("/proc/cpuinfo").read().split("\n\n"):
+            d = {}
+            for ln in blk.splitlines():
+                if ":" in ln:
+                    k, v = ln.split(":", 1)
+                    d[k.strip()] = v.strip()
+            if "processor" in d and int(d["processor"]) in aff:
+                model = d.get("model name", model)
+                phys.add((d.get("physical id"), d.get("core id")))
     except OSError:
         pass
-    return "unknown"
+    return dict(cores=len(aff), cgroup_quota_cpus=quota, physical_cores=len(phys) or len(aff), model=model,
+                nproc=os.cpu_count())
+
+
+def cpu_impl(threads):
+    import oracle
+
+    if oracle.ref_available():
+        return oracle.Reference(threads=threads), "reference"
+    return oracle.Oracle("auto", threads=threads), "port"
+
+
+def cpu_invert(impl, kind, n, p, A):
+    if kind == "reference":
+        return impl.invert(n, p, A)[1]
+    return impl.invert(n, p, A)[1]
 
 
 def cpu_baseline(lx, p, seed, threads, reps=5):
-    """The reference's CPU path (oracle/_ref) on the same matrix (instance 0):
-    one warm-up inversion, then the median of `reps` (each on a fresh copy)."""
-    import oracle
+    """The reference's CPU path (oracle/_ref) on the same matrix (instance 0),
+    BLAS threads = `threads`: one warm-up inversion, then the median of
+    `reps` (each on a fresh copy)."""
     from bsofi_amd import dqmc
 
-    if oracle.ref_available():
-        impl = oracle.Reference(threads=threads)
-        kind = "reference"
-    else:
-        impl = oracle.Oracle("auto", threads=threads)
-        kind = "port"
+    impl, kind = cpu_impl(threads)
     A0, meta = dqmc.dqmc_matrix(lx, p, seed=seed, instance=0)
     n = meta["n"]
     times, info = [], 0
     for r in range(reps + 1):
         A = A0.copy(order="F")
         t0 = time.perf_counter()
-        if kind == "reference":
-            tau, info = impl.invert(n, p, A)
-        else:
-            tau, info, _ = impl.invert(n, p, A)
+        info = cpu_invert(impl, kind, n, p, A)
         if r > 0:
             times.append(time.perf_counter() - t0)
     return dict(seconds=float(np.median(times)), kind=kind, info=info, reps=reps)
+
+
+def cpu_batch_baseline(specs, threads, seed):
+    """A batched workload on the host, both ways the reference's CPU path can
+    run it: threaded (BLAS threads = all cores, one matrix after another) and
+    throughput (one matrix per core at a time, one BLAS thread each, `threads`
+    host threads calling the reference concurrently).  specs: (lx, p, U,
+    instance).  Inputs are built before the timing; returns inversions/s of
+    each mode over the sample."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from bsofi_amd import dqmc
+
+    mats = [(dqmc.dqmc_matrix(lx, p, U=U, seed=seed, instance=inst)[0], lx * lx, p) for lx, p, U, inst in specs]
+    out = {}
+    for mode in ("threaded", "throughput"):
+        impl, kind = cpu_impl(threads if mode == "threaded" else 1)
+        work = [A.copy(order="F") for A, _, _ in mats]
+        t0 = time.perf_counter()
+        if mode == "threaded":
+            infos = [cpu_invert(impl, kind, n, p, A) for A, (_, n, p) in zip(work, mats)]
+        else:
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                infos = list(ex.map(lambda a: cpu_invert(impl, kind, a[1][1], a[1][2], a[0]), zip(work, mats)))
+        dt = time.perf_counter() - t0
+        out[mode] = dict(inversions_per_s=len(mats) / dt, seconds=dt, info_nonzero=sum(1 for i in infos if i))
+        del work
+    best = max(out, key=lambda m: out[m]["inversions_per_s"])
+    return dict(out, best=best, kind=kind, sample=len(mats))
+
+
+def family_roofline(r, flops):
+    """Per-inversion time, launches and achieved FP64 rate of each kernel
+    family of the plain run (flops.families: algorithmic flops of the panel
+    QRs, the reflector applications, BSTRI and the BSOI scan)."""
+    fl = flops.families(r["n"], r["p"], r.get("nb", 32))
+    out = {}
+    for name, (ms, cnt) in r.get("families", {}).items():
+        e = {"ms_per_inversion": round(ms, 4), "launches": cnt}
+        if name in fl and ms > 0:
+            e["flops"] = fl[name]
+            e["tflops"] = round(fl[name] / (ms * 1e-3) / 1e12, 3)
+            e["frac"] = round(fl[name] / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12), 4)
+        out[name] = e
+    return out
+
+
+def cpu_batch_line(cb, sample, threads):
+    best = cb[cb["best"]]
+    return {
+        "value": round(best["inversions_per_s"], 3),
+        "unit": "inversions/s",
+        "mode": cb["best"],
+        "threaded_inversions_per_s": round(cb["threaded"]["inversions_per_s"], 3),
+        "throughput_inversions_per_s": round(cb["throughput"]["inversions_per_s"], 3),
+        "cores": threads,
+        "kind": cb["kind"],
+        "sample": f"{sample}: threaded = MKL {threads} threads, one matrix after another; throughput = "
+                  f"{threads} host threads, one matrix each at a time, MKL 1 thread",
+        "info_nonzero": cb["threaded"]["info_nonzero"] + cb["throughput"]["info_nonzero"],
+    }
 
 
 def main():
@@ -436,7 +535,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed, args.nb, args.scan_mti)
+    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed, args.nb, args.scan_mti, families=True)
     n, p, N = r["n"], r["p"], r["N"]
     value = args.steps / r["wall"]
     tot_flops = flops.bsofi(n, p)
@@ -481,6 +580,7 @@ def main():
             "flops_per_launch": bsoi_flops,
             "avg_launch_ms": round(r["scan_ms"], 4),
             "launches": r["scan_launches"],
+            "families": family_roofline(r, flops),
         },
     }
     if args.batched:
@@ -553,8 +653,9 @@ def main():
             "info": list(e["info"]),
         }
     if args.cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline: {threads} threads")
+        host = host_info()
+        threads = args.cpu_threads or host["cores"]
+        log(f"[bench] cpu baseline: {threads} threads on {host}")
         c = cpu_baseline(args.lx, args.p, args.seed, threads)
         line["cpu_baseline"] = {
             "value": round(1.0 / c["seconds"], 5),
@@ -562,10 +663,20 @@ def main():
             "cores": threads,
             "kind": c["kind"],
             "sample": f"median of {c['reps']} full inversions (after one warm-up) of the same n={n}, p={p} "
-                      f"matrix (instance 0), MKL {threads} threads",
-            "cpu_model": cpu_model(),
+                      f"matrix (instance 0), MKL {threads} threads (every core this host exposes)",
+            "host": host,
+            "blas": "MKL (libmkl_rt)" if c["kind"] == "reference" else "the oracle's LAPACK back end",
             "gflops": round(tot_flops / c["seconds"] / 1e9, 2),
         }
+        if args.batched:
+            cb = cpu_batch_baseline([(8, 32, 4.0, 1000 + k) for k in range(64)], threads, args.seed)
+            line["cpu_baseline"]["batched_uniform"] = cpu_batch_line(cb, "64 of the 512 (n=64, p=32) matrices", threads)
+        if args.uneven:
+            from bsofi_amd import workload
+
+            cu = cpu_batch_baseline(list(workload.uneven_specs(args.uneven, args.seed)[:16]), threads, args.seed)
+            line["cpu_baseline"]["uneven"] = cpu_batch_line(cu, f"the first 16 of the {args.uneven} uneven matrices",
+                                                            threads)
     print(json.dumps(line), flush=True)
 
 

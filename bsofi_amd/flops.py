@@ -79,3 +79,29 @@ def bytes_min(n, L):
     """Write G, read the 2p non-zero blocks of A, write tau (8-byte words)."""
     N = n * L
     return 8.0 * (N * N + 2 * L * n * n + N)
+
+
+def families(n, L, nb=32):
+    """Algorithmic flops per inversion of each kernel family of this build's
+    plan (the numerators of the per-family roofline figures):
+
+    * panel -- the sub-panel QRs: dgeqrf(m, nbj) of every sub-panel of the
+      BSOF sweep (m = 2n - j nb rows, nbj <= nb columns);
+    * refl  -- block reflector applications, 4 m nbj w each (W = V^T C and
+      C -= V (T W), leading order): the BSOF updates of the w = w1 + w2
+      trailing and last-block-column columns, and BSOI's formation of the
+      explicit Q_k (applied to the identity, w = m);
+    * gemm  -- BSTRI, counted as the reference counts it (bstri(n, L));
+    * scan  -- BSOI, counted as the reference counts it (bsoi(n, L)).
+    """
+    panel = refl = 0.0
+    for k in range(L - 1):
+        ncols = 2 * n if k == L - 2 else n
+        for j in range(0, (ncols + nb - 1) // nb):
+            m = 2 * n - j * nb
+            nbj = min(nb, ncols - j * nb)
+            panel += dgeqrf(m, nbj)
+            w = (2 * n - j * nb - nbj) + (n if k < L - 2 else 0)
+            refl += 4.0 * m * nbj * w  # BSOF update
+            refl += 4.0 * m * nbj * m  # Q_k formation in BSOI
+    return {"panel": panel, "refl": refl, "gemm": bstri(n, L), "scan": bsoi(n, L)}
