@@ -238,10 +238,11 @@ def bench_batched(B, lx, p, count, reps, seed):
     return dict(ms=float(np.median(times)), resid=res, info_bad=sum(1 for i in infos if i != 0))
 
 
-def bench_uneven(B, count, seed, reps, tune_p, log_fn):
+def bench_uneven(B, count, seed, reps, tune_p, log_fn, save_tune=None):
     """The uneven mixed workload: `count` matrices with lattice side from
     {6, 8, 10, 12, 16}, p from U{10..80}, U from U[2, 6] (seeded draw).
-    (1) the autotuner picks nb / scan tile height per n on small batches;
+    (1) the autotuner picks nb, scan form, GEMM tile and reflector slabs per
+        workgroup per n, on trial batches of the workload's own matrices;
     (2) all matrices in ONE plan on the tuned context, captured in a HIP
         graph, `reps` executions on rebuilt inputs;
     (3) the plain path on the same matrices: one single-matrix plan each on
@@ -255,7 +256,10 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn):
     specs = workload.uneven_specs(count, seed)
     ctx = new_ctx(B)
     t0 = time.perf_counter()
-    table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=8, reps=2, log=log_fn)
+    table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=8, reps=2, log=log_fn,
+                          specs=specs)
+    if save_tune:
+        autotune.save(table, save_tune)
     tune_s = time.perf_counter() - t0
     log_fn(f"[bench] uneven: building {count} matrices")
     mats = workload.DeviceMatrices(B, ctx, specs, seed=seed)
@@ -502,6 +506,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--schedule", type=int, default=0,
                     help="BSOF schedule: 1 = one device-scheduled launch per plan, 0 = one launch per sweep step")
+    ap.add_argument("--save-tune", default=None, help="write the uneven run's autotuning table (JSON) here")
     ap.add_argument("--nb", type=int, default=0, help="sub-panel width for the plain run (0 = default)")
     ap.add_argument("--scan-mti", type=int, default=0, help="scan tile height / 16 for the plain run (0 = default)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -609,7 +614,7 @@ def main():
             "scan_ms": round(dg["scan_ms"], 3),
         }
     if args.uneven:
-        u = bench_uneven(B, args.uneven, args.seed, 2, 24, log)
+        u = bench_uneven(B, args.uneven, args.seed, 2, 24, log, args.save_tune)
         b_ips = args.uneven / (u["batched_ms"] * 1e-3)
         p_ips = args.uneven / (u["plain_ms"] * 1e-3)
         line["uneven"] = {
@@ -626,7 +631,8 @@ def main():
             "resid_ok": u["resid_ok"],
             "plain_resid_ok_sampled": u["plain_resid_ok"],
             "resid_max_over_bound": u["resid_max_rel"],
-            "autotune": {str(k): f"nb={v['nb']} scan_mti={v['scan_mti']}" for k, v in u["tune"].items()},
+            "autotune": {str(k): f"nb={v['nb']} scan_mti={v['scan_mti']} gemm_tile={v['gemm_tile']} "
+                                 f"refl_spw={v['refl_spw']}" for k, v in u["tune"].items()},
             "autotune_s": round(u["tune_s"], 1),
         }
     if args.serve:

@@ -95,6 +95,8 @@ def _load():
         "bsofi_ctx_cached_bytes": (C.c_size_t, [_vp]),
         "bsofi_ctx_set_lanes": (i, [_vp, i]),
         "bsofi_ctx_set_schedule": (i, [_vp, i]),
+        "bsofi_ctx_set_tuning_ex": (i, [_vp, i, i, i, i, i]),
+        "bsofi_ctx_get_tuning_ex": (i, [_vp, i, _ip, _ip, _ip, _ip]),
         "bsofi_ctx_set_wait_limit": (i, [_vp, d]),
         "bsofi_fp64_rate": (i, [_vp, i, _dp]),
         "bsofi_ctx_release_cache": (i, [_vp]),
@@ -289,11 +291,20 @@ class Context:
         if rc:
             raise RuntimeError(f"synchronize failed ({rc})")
 
-    def set_tuning(self, n, nb=0, scan_mti=0):
-        """Sub-panel width and scan rows/16 for block size n (0 = default)."""
-        rc = lib.bsofi_ctx_set_tuning(self.h, n, nb, scan_mti)
+    def set_tuning(self, n, nb=0, scan_mti=0, gemm_tile=0, refl_spw=0):
+        """Sub-panel width, scan form, BSTRI GEMM tile and reflector slabs per
+        workgroup for block size n (0 = default)."""
+        rc = lib.bsofi_ctx_set_tuning_ex(self.h, n, nb, scan_mti, gemm_tile, refl_spw)
         if rc:
-            raise ValueError(f"unsupported tuning n={n} nb={nb} scan_mti={scan_mti} ({rc})")
+            raise ValueError(f"unsupported tuning n={n} nb={nb} scan_mti={scan_mti} gemm_tile={gemm_tile} "
+                             f"refl_spw={refl_spw} ({rc})")
+
+    def get_tuning_ex(self, n):
+        v = [C.c_int(0) for _ in range(4)]
+        rc = lib.bsofi_ctx_get_tuning_ex(self.h, n, *[C.byref(x) for x in v])
+        if rc:
+            raise RuntimeError(f"get_tuning_ex failed ({rc})")
+        return tuple(x.value for x in v)
 
     def get_tuning(self, n):
         nb, mti = C.c_int(0), C.c_int(0)

@@ -10,9 +10,15 @@ DQMC matrices of that n under each supported pair
                 taller tiles reuse each Q_k fragment over more rows but
                 leave fewer workgroups for small batches,
 
-keeps the fastest (median of `reps` executions of one plan), and sets it on
-the context with bsofi_ctx_set_tuning, so plans created afterwards use it.
-The table can be saved as JSON and applied again without re-timing.
+keeps the fastest (median of `reps` executions of one plan), then does the
+same for two knobs that change only speed -- the BSTRI GEMM tile edge
+(32, 64, 128, or the launch-size rule) and the reflector applications'
+16-column slabs per workgroup (1, 2, 4, 8, or the launch-size rule) -- and
+sets the result on the context with bsofi_ctx_set_tuning_ex, so plans created
+afterwards use it.  Trial batches can be drawn from the workload's own
+(n, p, U) draw.  The table can be saved as JSON and applied again without
+re-timing (replacing the reference's tune/hybrid/tune_hybrid.c, which timed
+its CPU/GPU split constants).
 """
 from __future__ import annotations
 
@@ -60,33 +66,67 @@ def time_plan(B, ctx, descs, reps, rebuild):
         plan.close()
 
 
-def tune(B, ctx, sides, p=24, batch=8, reps=3, log=None):
-    """Tune every lattice side in `sides` (n = side^2).  Returns
-    {n: {"nb": .., "scan_mti": .., "ms": .., "all": {"nb,mti": ms}}}."""
+GEMM_TILES = (0, 32, 64, 128)
+REFL_SPWS = (0, 1, 2, 4, 8)
+
+
+def tune(B, ctx, sides, p=24, batch=8, reps=3, log=None, specs=None, margin=0.03):
+    """Tune every lattice side in `sides` (n = side^2) on the device.
+
+    Trial batch per n: with `specs` (the workload's own (lx, p, U, instance)
+    draw), up to `batch` of its matrices with that lattice side, so the p
+    values and U of the workload are the ones timed; otherwise `batch`
+    matrices of order p.  Coordinate search, keeping the fastest at each
+    stage: (nb, scan form) over candidates(n), then the BSTRI GEMM tile, then
+    the reflector slabs per workgroup (these two keep their launch-size rule
+    unless a fixed value is faster by more than `margin`).  None of the knobs
+    changes the bits.
+    Returns {n: {"nb", "scan_mti", "gemm_tile", "refl_spw", "ms", "all"}}."""
     table = {}
     for lx in sides:
         n = lx * lx
-        mats = DeviceMatrices(B, ctx, [(lx, p, 4.0, 9000 + k) for k in range(batch)], keep_blocks=False)
+        if specs:
+            trial = [tuple(sp) for sp in specs if sp[0] == lx][:batch]
+        else:
+            trial = [(lx, p, 4.0, 9000 + k) for k in range(batch)]
+        mats = DeviceMatrices(B, ctx, trial, keep_blocks=False)
         res = {}
-        for nb, mti in candidates(n):
-            ctx.set_tuning(n, nb, mti)
-            t0 = time.perf_counter()
-            res[(nb, mti)] = time_plan(B, ctx, mats.descs(), reps, mats.rebuild)
-            mats.rebuild()
-            if log:
-                log(f"[tune] n={n} nb={nb} scan_mti={mti}: {res[(nb, mti)]:.3f} ms "
-                    f"({time.perf_counter() - t0:.1f} s)")
-        best = min(res, key=res.get)
+
+        def trial_ms(nb, mti, tile, spw):
+            key = (nb, mti, tile, spw)
+            if key not in res:
+                ctx.set_tuning(n, nb, mti, tile, spw)
+                t0 = time.perf_counter()
+                res[key] = time_plan(B, ctx, mats.descs(), reps, mats.rebuild)
+                mats.rebuild()
+                if log:
+                    log(f"[tune] n={n} nb={nb} scan_mti={mti} gemm_tile={tile} refl_spw={spw}: "
+                        f"{res[key]:.3f} ms ({time.perf_counter() - t0:.1f} s)")
+            return res[key]
+
+        def pick(keys):
+            # the launch-size rule (knob 0) stays unless a fixed value beats
+            # it by more than `margin`: the rule adapts to the width of a
+            # real batch's launches, which a small trial batch does not show
+            base = keys[0]
+            best_k = min(keys, key=lambda k: trial_ms(*k))
+            return best_k if trial_ms(*best_k) < (1.0 - margin) * trial_ms(*base) else base
+
+        best = min(((nb, mti, 0, 0) for nb, mti in candidates(n)), key=lambda k: trial_ms(*k))
+        best = pick([(best[0], best[1], t, 0) for t in GEMM_TILES])
+        best = pick([(best[0], best[1], best[2], s) for s in REFL_SPWS])
         ctx.set_tuning(n, *best)
-        table[n] = {"nb": best[0], "scan_mti": best[1], "ms": res[best],
-                    "all": {f"{a},{b}": round(v, 4) for (a, b), v in res.items()}}
+        table[n] = {"nb": best[0], "scan_mti": best[1], "gemm_tile": best[2], "refl_spw": best[3], "ms": res[best],
+                    "trial": [list(t[:2]) for t in trial],
+                    "all": {",".join(str(v) for v in k): round(v, 4) for k, v in res.items()}}
         mats.release()
     return table
 
 
 def apply(ctx, table):
     for n, row in table.items():
-        ctx.set_tuning(int(n), int(row["nb"]), int(row["scan_mti"]))
+        ctx.set_tuning(int(n), int(row["nb"]), int(row["scan_mti"]), int(row.get("gemm_tile", 0)),
+                       int(row.get("refl_spw", 0)))
 
 
 def save(table, path):

@@ -78,6 +78,8 @@ struct Geo {
   int n, p, lda, N, nb, S;
   int panel_variant;  // panel_factor_kernel instance
   int scan_mti, scan_ntw;
+  int gemm_tile;  // BSTRI GEMM tile edge: 0 = the launch-size rule, else 32, 64 or 128
+  int refl_spw;   // reflector slabs per workgroup: 0 = the launch-size rule, else 1, 2, 4 or 8
 };
 
 static int nb_for(int n) { return (2 * n <= 512) ? 32 : 16; }
@@ -86,7 +88,7 @@ static int nb_for(int n) { return (2 * n <= 512) ? 32 : 16; }
 // the sub-panel width nb of BSOF / BSOI and the rows per workgroup of the
 // BSOI scan (16 * scan_mti).  Zero keeps the default.
 struct Tune {
-  int nb = 0, scan_mti = 0;
+  int nb = 0, scan_mti = 0, gemm_tile = 0, refl_spw = 0;
 };
 
 // Sub-panel widths with a panel_factor_kernel instance for 2n rows.
@@ -153,6 +155,8 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
                    ? tune->scan_mti
                    : (n <= 256 ? (n >= 128 && n % 8 == 0 ? kScan4 : 2) : (n <= 512 ? 1 : kScan8W));
   g.scan_ntw = scan_ntw_for(n, g.scan_mti);
+  g.gemm_tile = tune ? tune->gemm_tile : 0;
+  g.refl_spw = tune ? tune->refl_spw : 0;
   return g;
 }
 
@@ -374,16 +378,19 @@ struct Planner {
     std::vector<ReflTask> t;
     std::vector<int> slabs;
     size_t total = 0;
-    void add(const ReflTask& q, int n) {
+    int spw_set = 0;  // smallest tuned slabs-per-workgroup among the tasks (0: none tuned)
+    void add(const ReflTask& q, int n, int tuned_spw = 0) {
       t.push_back(q);
       slabs.push_back(n);
       total += (size_t)n;
+      if (tuned_spw > 0) spw_set = spw_set ? std::min(spw_set, tuned_spw) : tuned_spw;
     }
   };
   void emit_refl(const ReflRound& R) {
     int variant = refl_variant(R.t);
     const size_t round = cus * (variant == 101 ? 2 : 1);  // multi-slab workgroups per CU (VGPRs, LDS)
-    const int spw = variant <= 104 ? (int)std::min<size_t>(8, (R.total + round - 1) / round) : 1;
+    int spw = variant <= 104 ? (int)std::min<size_t>(8, (R.total + round - 1) / round) : 1;
+    if (R.spw_set && variant <= 104) spw = R.spw_set;  // the autotuner's choice for these block sizes
     if (spw > 1) variant += 100;
     std::vector<Item> items;
     for (size_t k = 0; k < R.t.size(); ++k)
@@ -458,7 +465,7 @@ struct Planner {
           q.w1 = w1;
           q.w2 = w2;
           q.trans = 1;
-          rr.add(q, (w1 + w2 + 15) / 16);
+          rr.add(q, (w1 + w2 + 15) / 16, g.refl_spw);
         }
       }
       for (auto& kv : pt) emit(F_PANEL, panel_instance(kv.first, kv.second), kv.second, pi[kv.first]);
@@ -608,15 +615,18 @@ struct Planner {
   // plain inversion was 0.8 % slower).
   static bool gemm_big(int M, int N) { return (long long)(M / 128) * (N / 128) >= 256; }
 
+  // A task's tile is the launch-size rule's (64, or 128 for big products;
+  // 64-tile launches of fewer tiles than CUs become 32-tile launches) unless
+  // its matrix's tuning forces 32, 64 or 128.  Every tile size keeps each
+  // element's k order, so the choice never changes the bits.
   struct GemmBatch {
-    std::vector<GemmTask> t64, t128;
-    std::vector<Item> i64, i128;
-    void add(GemmTask t) {
+    std::vector<GemmTask> t64, t128, f32, f64;
+    std::vector<Item> i64, i128, j32, j64;
+    void add(GemmTask t, int forced = 0) {
       if (t.M <= 0 || t.N <= 0) return;
-      const bool big = gemm_big(t.M, t.N);
-      const int tile = big ? 128 : 64;
-      auto& tasks = big ? t128 : t64;
-      auto& items = big ? i128 : i64;
+      const int tile = forced ? forced : (gemm_big(t.M, t.N) ? 128 : 64);
+      auto& tasks = tile == 128 ? t128 : tile == 32 ? f32 : (forced ? f64 : t64);
+      auto& items = tile == 128 ? i128 : tile == 32 ? j32 : (forced ? j64 : i64);
       t.tiles_n = (t.N + tile - 1) / tile;
       const int tm = (t.M + tile - 1) / tile;
       const int id = (int)tasks.size();
@@ -643,6 +653,8 @@ struct Planner {
       emit(F_GEMM, 0, b.t64, b.i64);
     }
     emit(F_GEMM, 1, b.t128, b.i128);
+    emit(F_GEMM, 2, b.f32, b.j32);
+    emit(F_GEMM, 0, b.f64, b.j64);
   }
 
   static GemmTask gt(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
@@ -702,8 +714,10 @@ struct Planner {
           double* Bii = blk(M, i, i);
           double* Yi = M.Y + (size_t)i * g.n * 32;
           b1.add(gt(Bii + (size_t)32 * J * g.lda, g.lda, Bii + (size_t)32 * J * (g.lda + 1), g.lda, Yi, g.n, 32 * J,
-                    nbJ, nbJ, 1.0, 0, 1));
-          b2.add(gt(Bii, g.lda, Yi, g.n, Bii + (size_t)32 * J * g.lda, g.lda, 32 * J, nbJ, 32 * J, -1.0, 1, 0));
+                    nbJ, nbJ, 1.0, 0, 1),
+                 g.gemm_tile);
+          b2.add(gt(Bii, g.lda, Yi, g.n, Bii + (size_t)32 * J * g.lda, g.lda, 32 * J, nbJ, 32 * J, -1.0, 1, 0),
+                 g.gemm_tile);
         }
       }
       emit_gemm(b1);
@@ -718,11 +732,14 @@ struct Planner {
         const Geo& g = M.g;
         const size_t n2 = (size_t)g.n * g.n;
         for (int i = 0; i <= g.p - 2; ++i)
-          b1.add(gt(blk(M, i, i), g.lda, blk(M, i, i + 1), g.lda, M.W + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0));
+          b1.add(gt(blk(M, i, i), g.lda, blk(M, i, i + 1), g.lda, M.W + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0),
+                 g.gemm_tile);
         for (int i = 0; i <= g.p - 3; ++i) {
           b1.add(gt(blk(M, i, g.p - 1), g.lda, blk(M, g.p - 1, g.p - 1), g.lda, M.Z1 + i * n2, g.n, g.n, g.n, g.n,
-                    1.0, 0, 1));
-          b2.add(gt(blk(M, i, i), g.lda, M.Z1 + i * n2, g.n, M.Z2 + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0));
+                    1.0, 0, 1),
+                 g.gemm_tile);
+          b2.add(gt(blk(M, i, i), g.lda, M.Z1 + i * n2, g.n, M.Z2 + i * n2, g.n, g.n, g.n, g.n, -1.0, 1, 0),
+                 g.gemm_tile);
         }
       }
       emit_gemm(b1);
@@ -745,7 +762,7 @@ struct Planner {
           q.ldd = g.n;
           q.dcol0 = (g.p - 2 - i) * g.n;
         }
-        b.add(q);
+        b.add(q, g.gemm_tile);
       }
       emit_gemm(b);
     }
@@ -808,7 +825,7 @@ struct Planner {
           q.w1 = n2 - j * g.nb;
           q.w2 = 0;
           q.trans = 0;
-          rr.add(q, (q.w1 + 15) / 16);
+          rr.add(q, (q.w1 + 15) / 16, g.refl_spw);
         }
       }
       emit_refl(rr);
@@ -1105,6 +1122,34 @@ int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti) {
     ctx->tune.erase(n);
   else
     ctx->tune[n] = t;
+  return 0;
+}
+
+int bsofi_ctx_set_tuning_ex(bsofi_ctx_t ctx, int n, int nb, int scan_mti, int gemm_tile, int refl_spw) {
+  if (!ctx || n < 1 || n > BSOFI_MAX_N || nb < 0 || scan_mti < 0) return -1;
+  if (gemm_tile != 0 && gemm_tile != 32 && gemm_tile != 64 && gemm_tile != 128) return BSOFI_ERR_UNSUPPORTED;
+  if (refl_spw != 0 && refl_spw != 1 && refl_spw != 2 && refl_spw != 4 && refl_spw != 8) return BSOFI_ERR_UNSUPPORTED;
+  Tune t;
+  t.nb = nb;
+  t.scan_mti = scan_mti;
+  t.gemm_tile = gemm_tile;
+  t.refl_spw = refl_spw;
+  const Geo g = geometry(n, 2, 2 * n, &t);
+  if (!nb_supported(n, g.nb) || !scan_ok(n, g.scan_mti, g.scan_ntw)) return BSOFI_ERR_UNSUPPORTED;
+  if (nb == 0 && scan_mti == 0 && gemm_tile == 0 && refl_spw == 0)
+    ctx->tune.erase(n);
+  else
+    ctx->tune[n] = t;
+  return 0;
+}
+
+int bsofi_ctx_get_tuning_ex(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti, int* gemm_tile, int* refl_spw) {
+  if (!ctx || !nb || !scan_mti || !gemm_tile || !refl_spw || n < 1) return -1;
+  const Geo g = geometry(n, 2, 2 * n, ctx_tune(ctx, n));
+  *nb = g.nb;
+  *scan_mti = g.scan_mti;
+  *gemm_tile = g.gemm_tile;
+  *refl_spw = g.refl_spw;
   return 0;
 }
 

@@ -181,6 +181,15 @@ int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
  * *_SWITCH_*) these choices replace. */
 int bsofi_ctx_set_tuning(bsofi_ctx_t ctx, int n, int nb, int scan_mti);
 int bsofi_ctx_get_tuning(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti);
+/* The same with two more knobs: gemm_tile = edge of the BSTRI GEMM tiles
+ * (0: 128 where a product yields a full round of such tiles, else 64, and
+ * 32 for launches of fewer 64-tiles than CUs; or 32, 64, 128 fixed) and
+ * refl_spw = 16-column slabs per workgroup of the reflector applications up
+ * to m = 512 (0: as many as fit one round of workgroups, at most 8; or 1, 2,
+ * 4, 8 fixed; a launch that mixes block sizes takes the smallest setting).
+ * Neither changes the bits.  set_tuning keeps them at 0. */
+int bsofi_ctx_set_tuning_ex(bsofi_ctx_t ctx, int n, int nb, int scan_mti, int gemm_tile, int refl_spw);
+int bsofi_ctx_get_tuning_ex(bsofi_ctx_t ctx, int n, int* nb, int* scan_mti, int* gemm_tile, int* refl_spw);
 int bsofi_ctx_clear_tuning(bsofi_ctx_t ctx);
 /* Lanes of batch plans created on the context afterwards: 2 (the default)
  * splits a plan of two or more matrices into two groups of about equal work

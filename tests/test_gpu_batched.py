@@ -559,3 +559,35 @@ def test_device_schedule_wait_limit_reports_error(B):
     assert info0 == [0, 0] and info1 == [0, 0]
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+
+
+def test_tuning_knobs_same_bits(B, eps):
+    """The BSTRI GEMM tile (32, 64, 128 or the launch-size rule) and the
+    reflector slabs per workgroup (1, 2, 4, 8 or the rule) change only speed:
+    every value gives the default's bits, for a batch of mixed block sizes
+    whose steps span several tile and slab choices."""
+    sp = [(8, 8), (8, 6), (12, 4), (16, 3), (6, 9), (4, 12)]
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=300 + k)[0] for k, (lx, p) in enumerate(sp)]
+    sizes = [(lx * lx, p) for lx, p in sp]
+    ref, info = _run_plan(B, inputs, sizes, 0)
+    assert info == [0] * len(sp)
+    for k, (A0, (n, p)) in enumerate(zip(inputs, sizes)):
+        assert oracle.residual(A0, ref[k]) <= 100 * n * p * eps
+    import torch
+
+    for knob in [("gemm_tile", t) for t in (32, 64, 128)] + [("refl_spw", s) for s in (1, 2, 4, 8)]:
+        print(f"knob {knob}", flush=True)
+        ctx = B.Context(0)
+        for n, _ in sizes:
+            ctx.set_tuning(n, **{knob[0]: knob[1]})
+            assert ctx.get_tuning_ex(n)[2 if knob[0] == "gemm_tile" else 3] == knob[1]
+        devs = [to_dev(A0) for A0 in inputs]
+        taus = [torch.zeros(n * p, dtype=torch.float64, device="cuda") for n, p in sizes]
+        torch.cuda.synchronize()
+        assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)
+                                      for d, t, (n, p) in zip(devs, taus, sizes)]) == [0] * len(sp)
+        for k, (d, (n, p)) in enumerate(zip(devs, sizes)):
+            assert np.array_equal(from_dev(d, n * p), ref[k]), f"{knob}: matrix {k} differs"
+        with pytest.raises(ValueError):
+            ctx.set_tuning(64, gemm_tile=48)
+        ctx.close()
