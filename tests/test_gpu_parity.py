@@ -315,3 +315,26 @@ def test_debug_gemm_matches_numpy(B):
         ref = -1.5 * (np.triu(A) if am else A) @ (np.triu(Bm) if bm else Bm) + 0.5 * Cm
         assert np.abs(dC.cpu().numpy().T - ref).max() <= 1e-13 * np.abs(ref).max()
     ctx.close()
+
+
+def test_nan_input_runs_through(B, O):
+    """A NaN in A, which neither the reference nor LAPACK checks for: the GPU
+    path ends with info = 0, as the reference's LAPACK path does on the same
+    input, and the NaN reaches G; nothing hangs on it (the panel's norm
+    checks send a NaN column down LAPACK's scaled dlarfg path, whose loops
+    are bounded)."""
+    A0, m = dqmc.dqmc_matrix(3, 4)
+    n, p = m["n"], 4
+    N = n * p
+    A0 = A0.copy(order="F")
+    A0[5, 2] = np.nan
+    G = A0.copy(order="F")
+    _, info_ref, _ = O.invert(n, p, G)
+    A = A0.copy(order="F")
+    tau = np.zeros(N)
+    h = B.HandleAPI(n, p, N)
+    work = np.zeros(max(h.lwork(), 1))
+    assert h.bsoftri(A, tau, work) == info_ref == 0
+    assert h.bsoi(A, tau, work) == 0
+    h.finalize()
+    assert np.isnan(A).any() and np.isnan(G).any()
