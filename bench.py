@@ -135,7 +135,7 @@ def scan_kernel_name(n, scan_mti=0):
     return "bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form == 9 else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)"
 
 
-def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, families=False):
+def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, families=False, graph=False):
     import torch
 
     phases = B.PHASE_ALL if phases is None else phases
@@ -148,6 +148,14 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, fa
     ds = DeviceSet(B, ctx, lx, p, nbuf, seed)
     plans = [B.Plan(ctx, [ds.desc(k)], phases=phases) for k in range(nbuf)]
     log(f"[bench] plan: {plans[0].num_launches} launches per inversion")
+    if graph:
+        # each plan is captured into a HIP graph on its first execution: do
+        # that before the timing, then rebuild the inputs it overwrote
+        for pl in plans:
+            pl.use_graph(True)
+            pl.execute()
+        ctx.synchronize()
+        ds.rebuild()
     # warm-up on buffer `steps` (rebuilt between warm-up steps, outside timing)
     for w in range(warmup):
         plans[steps].execute()
@@ -155,7 +163,7 @@ def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, fa
         ds.rebuild([steps])
         log(f"[bench] warmup {w + 1}/{warmup} done")
     for pl in plans[:steps]:
-        pl.time_family("scan")
+        pl.time_family(-1 if graph else "scan")
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     ctx.synchronize()
@@ -507,6 +515,7 @@ def main():
     ap.add_argument("--schedule", type=int, default=0,
                     help="BSOF schedule: 1 = one device-scheduled launch per plan, 0 = one launch per sweep step")
     ap.add_argument("--save-tune", default=None, help="write the uneven run's autotuning table (JSON) here")
+    ap.add_argument("--graph", type=int, default=0, help="replay each plain plan as a HIP graph (captured beforehand)")
     ap.add_argument("--nb", type=int, default=0, help="sub-panel width for the plain run (0 = default)")
     ap.add_argument("--scan-mti", type=int, default=0, help="scan tile height / 16 for the plain run (0 = default)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -540,7 +549,8 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
-    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed, args.nb, args.scan_mti, families=True)
+    r = bench_plain(B, args.lx, args.p, args.steps, args.warmup, args.seed, args.nb, args.scan_mti, families=True,
+                    graph=bool(args.graph))
     n, p, N = r["n"], r["p"], r["N"]
     value = args.steps / r["wall"]
     tot_flops = flops.bsofi(n, p)
