@@ -264,8 +264,10 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn, save_tune=None):
     specs = workload.uneven_specs(count, seed)
     ctx = new_ctx(B)
     t0 = time.perf_counter()
-    table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=8, reps=2, log=log_fn,
-                          specs=specs)
+    # trial batch per block size: every matrix of that size in the workload
+    # (a smaller batch makes different launches and picks different forms)
+    table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=len(specs), reps=1,
+                          log=log_fn, specs=specs)
     if save_tune:
         autotune.save(table, save_tune)
     tune_s = time.perf_counter() - t0
