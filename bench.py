@@ -131,8 +131,9 @@ class DeviceSet:
 def scan_kernel_name(n, scan_mti=0):
     """The BSOI scan kernel a plan on block size n runs (the library's
     default scan form unless scan_mti overrides it)."""
-    form = scan_mti or (9 if 128 <= n <= 256 and n % 8 == 0 else (2 if n <= 256 else 1))
-    return "bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form == 9 else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)"
+    form = scan_mti or (10 if 128 <= n <= 256 and n % 4 == 0 else (2 if n <= 256 else 1))
+    return ("bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form in (9, 10)
+            else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)")
 
 
 def bench_plain(B, lx, p, steps, warmup, seed, nb=0, scan_mti=0, phases=None, families=False, graph=False):

@@ -33,11 +33,12 @@ from .workload import DeviceMatrices
 def candidates(n):
     nbs = [nb for nb, ok in ((32, 2 * n <= 512), (16, True), (8, True)) if ok][:2]
     # scan forms: 16 * m rows on the 16x16x4 MFMA with 4 waves (m = 1, 2, 4),
-    # 8: 16 rows with 8 waves (n > 512), or 9: 32 rows on the 4x4x4 MFMA
-    # (from n = 128: below, its one workgroup per CU starves real batches,
-    # which the tuner's small trial batch does not show)
+    # 8: 16 rows with 8 waves (n > 512), or 10: 32 rows on the 4x4x4 MFMA
+    # with operands loaded two k-steps ahead (4 | n).  Form 9, the same scan
+    # with two buffers (8 | n), gives the same bits and ran slower at every
+    # size measured, so the tuner leaves it out.
     mtis = [m for m, ok in ((1, n <= 512), (2, n <= 256), (4, n <= 128), (8, n > 512),
-                            (9, 128 <= n <= 256 and n % 8 == 0)) if ok]
+                            (10, n <= 256 and n % 4 == 0 and n >= 8)) if ok]
     return [(nb, m) for nb in nbs for m in mtis]
 
 

@@ -1392,9 +1392,15 @@ __global__ __launch_bounds__(64 * NW) void bsoi_scan_kernel(const ScanTask* __re
 // waves per SIMD with half the tiles each, as each A value then feeds 8
 // MFMAs).  The carry sits in LDS with row 4 g + y at y * 8 + g, so that a
 // lane's 8 A values of a k-step are contiguous.
-template <int NTW, int NW>
+//
+// DEPTH is the number of register buffers of the k-loop: 2 loads each
+// k-step's operands one step ahead, 3 two steps ahead (then every lane's
+// Q_k^T loads address one uniform row base plus a fixed per-lane offset).
+// Both run the same MFMAs in the same order, so the sums are the same.
+template <int NTW, int NW, int DEPTH = 2>
 __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* __restrict__ tasks,
                                                                 const Item* __restrict__ items, int ldcar) {
+  static_assert(DEPTH == 2 || DEPTH == 3, "two or three operand buffers");
   constexpr int NT = 64 * NW;
   constexpr int MT = 32;
   constexpr int XLD = 36;  // column stride of Xs: the A-fragment reads hit 16 distinct bank pairs
@@ -1423,14 +1429,9 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
                                      (__attribute__((address_space(3))) void*)(Xs + kk * XLD), 4, 0, 0);
   };
 
-  // rows of this lane's A values: R0 + 4 g + y
-  int rblk[8], rin[8];
-#pragma unroll
-  for (int g = 0; g < 8; ++g) {
-    const int row = R0 + 4 * g + y;
-    rblk[g] = row < N ? row / n : p;
-    rin[g] = row - rblk[g] * n;
-  }
+  // This lane's A values are rows R0 + 4 g + y.  Row i of X_{r,k} takes part
+  // in k-step column kk (of block column k) iff i < N and i lies in block row
+  // k or above, on or above the diagonal: i <= min(k n + kk, N - 1).
 
   for (int e = tid; e < n * MT; e += NT) {
     const int i = e % MT, kk = e / MT;
@@ -1457,8 +1458,12 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   // divide evenly over the waves, a wave's last tiles lie beyond 2n: they
   // read the last column (in bounds) and their results are never stored.
   int bcol[NTW];
+  unsigned boff[NTW];  // byte offset of (row x, column bcol[b]) in a 4-row band of Q_k^T
 #pragma unroll
-  for (int b = 0; b < NTW; ++b) bcol[b] = min((wave + NW * b) * 16 + l16, n2 - 1);
+  for (int b = 0; b < NTW; ++b) {
+    bcol[b] = min((wave + NW * b) * 16 + l16, n2 - 1);
+    boff[b] = (unsigned)(x * n2 + bcol[b]) * 8u;
+  }
   for (int k = p - 2; k >= kstop; --k) {
     const gptr<const double> Qk = glb(t.Q) + (size_t)k * t.qstride;
     double acc[8][NTW];
@@ -1470,14 +1475,14 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
     // Two register buffers: the loads of k-step s + 1 (Q_k^T from L2, the
     // A fragments from LDS) are in flight while the 8 NTW MFMAs of step s
     // run; with one wave per SIMD nothing else covers their latency, which
-    // is why X_{r,k} comes from LDS and not from HBM.  With 8 | n (and the
+    // is why X_{r,k} comes from LDS and not from HBM.  With 4 | n (and the
     // clamped columns) every index of the carry part and of Q_k^T is in
     // range, so only the X part of a workgroup whose rows reach the diagonal
     // block (or the matrix end) needs masks, applied to the A values just
     // before the MFMAs (bit g: row group g).  (A select right after a load,
     // or a predicated load, makes the compiler wait for the loads early.)
-    double af[2][8], bf[2][NTW];
-    int msk[2];
+    double af[DEPTH][8], bf[DEPTH][NTW];
+    int msk[DEPTH];
     auto step = [&](int u, auto masked_tag) {
       constexpr bool MASKED = decltype(masked_tag)::value;
       double a[8];
@@ -1491,21 +1496,71 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
 #pragma unroll
         for (int g = 0; g < 8; ++g) acc[g][b] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[g], bf[u][b], acc[g][b], 0, 0, 0);
     };
-    // n / 4 k-steps, an even number (8 | n): the loop body has no branch
+    // n / 4 k-steps, in pairs with two buffers (8 | n), any number >= 2 with
+    // three (4 | n): the loop body has no branch
     // around its loads, so the wait before each step can leave the next
     // step's loads in flight.
     auto run_part = [&](auto load, auto masked_tag) {
-      load(0, 0);
-      int k0 = 0;
-      for (; k0 + 8 < n; k0 += 8) {
+      if constexpr (DEPTH == 2) {
+        load(0, 0);
+        int k0 = 0;
+        for (; k0 + 8 < n; k0 += 8) {
+          load(1, k0 + 4);
+          step(0, masked_tag);
+          load(0, k0 + 8);
+          step(1, masked_tag);
+        }
         load(1, k0 + 4);
         step(0, masked_tag);
-        load(0, k0 + 8);
         step(1, masked_tag);
+      } else {
+        // S = n / 4 >= 2 k-steps; step s uses buffer s mod 3, loaded two
+        // steps ahead.  At the loop head buffers 0 and 1 hold steps s, s + 1.
+        const int S = n >> 2;
+        load(0, 0);
+        load(1, 4);
+        int s = 0;
+        for (; s + 5 <= S; s += 3) {
+          load(2, 4 * s + 8);
+          step(0, masked_tag);
+          load(0, 4 * s + 12);
+          step(1, masked_tag);
+          load(1, 4 * s + 16);
+          step(2, masked_tag);
+        }
+        const int rem = S - s;  // 2, 3 or 4
+        if (rem == 2) {
+          step(0, masked_tag);
+          step(1, masked_tag);
+        } else if (rem == 3) {
+          load(2, 4 * s + 8);
+          step(0, masked_tag);
+          step(1, masked_tag);
+          step(2, masked_tag);
+        } else {
+          load(2, 4 * s + 8);
+          step(0, masked_tag);
+          load(0, 4 * s + 12);
+          step(1, masked_tag);
+          step(2, masked_tag);
+          step(0, masked_tag);
+        }
       }
-      load(1, k0 + 4);
-      step(0, masked_tag);
-      step(1, masked_tag);
+    };
+    // Operand b of the 4-row band of Q_k^T that starts at row k0 of Q (Q_k
+    // itself, or its rows n .. 2n - 1 for the carry part, qrow0 = n).  The
+    // three-buffer form reads through a buffer resource over Q_k: a uniform
+    // band offset in an SGPR and a fixed per-lane offset, no per-step address
+    // arithmetic (out-of-range offsets read 0; none occur with 4 | n).
+    const __amdgpu_buffer_rsrc_t qrs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(t.Q + (size_t)k * t.qstride), (short)0, 8 * n2 * n2, 0x00020000);
+    auto qload = [&](gptr<const double> Q, int qrow0, int k0, int b) -> double {
+      if constexpr (DEPTH == 2) {
+        return Q[(size_t)(k0 + x) * n2 + bcol[b]];
+      } else {
+        return __builtin_bit_cast(double,
+                                  __builtin_amdgcn_raw_buffer_load_b64(qrs, boff[b], 8 * (qrow0 + k0) * n2, 0));
+      }
     };
     if (rbmin <= k) {
       // [X_{r,k}] part: K rows 0..n-1 of Q_k^T, X_{r,k} from Xs
@@ -1517,7 +1572,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
 #pragma unroll
               for (int g = 0; g < 8; ++g) af[u][g] = xr[4 * g];
 #pragma unroll
-              for (int b = 0; b < NTW; ++b) bf[u][b] = Qk[(size_t)kk * n2 + bcol[b]];
+              for (int b = 0; b < NTW; ++b) bf[u][b] = qload(Qk, 0, k0, b);
             },
             std::false_type{});
       } else {
@@ -1525,14 +1580,13 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
             [&](int u, int k0) {
               const int kk = k0 + x;
               const double* xr = Xs + kk * XLD + y;
-              int mk = 0;
 #pragma unroll
-              for (int g = 0; g < 8; ++g) {
-                af[u][g] = xr[4 * g];
-                mk |= (rblk[g] <= k && !(rblk[g] == k && rin[g] > kk)) << g;
-              }
+              for (int g = 0; g < 8; ++g) af[u][g] = xr[4 * g];
+              // bit g: 4 g <= lim, i.e. row groups 0 .. lim / 4
+              const int lim = min(k * n + kk, N - 1) - R0 - y;
+              const int mk = lim < 0 ? 0 : (lim >= 28 ? 0xFF : (1 << (lim / 4 + 1)) - 1);
 #pragma unroll
-              for (int b = 0; b < NTW; ++b) bf[u][b] = Qk[(size_t)kk * n2 + bcol[b]];
+              for (int b = 0; b < NTW; ++b) bf[u][b] = qload(Qk, 0, k0, b);
               msk[u] = mk;
             },
             std::true_type{});
@@ -1548,7 +1602,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
 #pragma unroll
             for (int g = 0; g < 8; ++g) af[u][g] = cr[g];
 #pragma unroll
-            for (int b = 0; b < NTW; ++b) bf[u][b] = Qc[(size_t)kk * n2 + bcol[b]];
+            for (int b = 0; b < NTW; ++b) bf[u][b] = qload(Qc, n, k0, b);
           },
           std::false_type{});
     }

@@ -50,6 +50,9 @@ static void planner_shapes() {
         CHECK(bsofi_ctx_set_lanes(ctx, lanes) == 0);
         CHECK(bsofi_ctx_set_tuning_ex(ctx, 64, 16, 4, 32, 2) == 0);
         CHECK(bsofi_ctx_set_tuning_ex(ctx, 64, 16, 4, 48, 0) == BSOFI_ERR_UNSUPPORTED);
+        CHECK(bsofi_ctx_set_tuning_ex(ctx, 36, 0, 10, 0, 0) == 0);  // three-buffer 4x4x4 scan: 4 | n
+        CHECK(bsofi_ctx_set_tuning_ex(ctx, 100, 0, 9, 0, 0) == BSOFI_ERR_UNSUPPORTED);  // two buffers: 8 | n
+        CHECK(bsofi_ctx_set_tuning_ex(ctx, 300, 0, 10, 0, 0) == BSOFI_ERR_UNSUPPORTED);  // n <= 256
         for (const auto& sh : shapes) {
           std::vector<bsofi_matrix_t> mats;
           for (auto [n, p] : sh) {

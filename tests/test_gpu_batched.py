@@ -351,9 +351,9 @@ def test_serving_alone_equals_mix(B, eps):
 def test_tuned_variants_match_oracle(B, eps):
     """Every (nb, scan_mti) pair the autotuner may pick gives the oracle's
     inverse: the nb = 16 panel instances, the 64-row scan tiles, the 16-row
-    scan tiles below n = 256, the 32-row 4x4x4 scan (scan_mti 9) where
-    8 divides n (from n = 128), and for n = 576 both sub-panel widths (8 and
-    16) under the 8-wave scan."""
+    scan tiles below n = 256, the 32-row 4x4x4 scan (scan_mti 10) where
+    4 divides n (n = 16, 64, 100, 144), and for n = 576 both sub-panel
+    widths (8 and 16) under the 8-wave scan."""
     import torch
 
     from bsofi_amd import autotune
@@ -382,8 +382,60 @@ def test_tuned_variants_match_oracle(B, eps):
         ctx.set_tuning(144, 0, 4)  # 64-row scan tiles need n <= 128
     ctx.clear_tuning()
     assert ctx.get_tuning(64) == (32, 2)
-    assert ctx.get_tuning(144) == (32, 9)  # the 4x4x4 scan for 8 | n in [128, 256]
+    assert ctx.get_tuning(144) == (32, 10)  # the three-buffer 4x4x4 scan for 4 | n in [128, 256]
     ctx.close()
+
+
+def test_scan4_three_buffer_form_same_bits(B):
+    """The 4x4x4 scan with operands loaded two k-steps ahead (scan_mti 10)
+    runs the same MFMAs in the same order as the two-buffer form (9): the
+    same bits, full and diagonal-only, for block sizes whose 32-row panels
+    straddle block rows (n = 16, 24, 144) and whose k-step counts S = n / 4
+    give every tail of the three-buffer loop (S - 2 mod 3 = 0: n = 8, 32;
+    1: n = 24, 144; 2: n = 16, 64, 256), in one batch.  Form 10 alone also
+    takes n = 4 mod 8 (n = 20, 36, 100: a half column tile at the end of
+    Q_k^T), against the oracle at 1e-10 and with its diagonal-only blocks
+    equal to the full inverse's."""
+    import torch
+
+    O = oracle.Oracle("auto", threads=1)
+    cases = [dqmc.dqmc_matrix(4, 7, instance=120)[0], dqmc.dqmc_matrix(8, 4, instance=121)[0],
+             dqmc.dqmc_matrix(12, 3, instance=122)[0], dqmc.dqmc_matrix(16, 2, instance=123)[0],
+             dqmc.dqmc_matrix(10, 3, instance=124)[0], dqmc.random_pcyclic(8, 5, seed=125),
+             dqmc.random_pcyclic(32, 4, seed=126), dqmc.random_pcyclic(24, 3, seed=127),
+             dqmc.dqmc_matrix(6, 6, instance=128)[0], dqmc.random_pcyclic(20, 4, seed=129)]
+    dims = [(16, 7), (64, 4), (144, 3), (256, 2), (100, 3), (8, 5), (32, 4), (24, 3), (36, 6), (20, 4)]
+    diag_only = B.PHASE_ALL | B.PHASE_DIAG_ONLY
+    out = {}
+    for form in (9, 10):
+        for phases in (B.PHASE_ALL, diag_only):
+            ctx = B.Context(0)
+            devs, taus = [], []
+            for A0, (n, p) in zip(cases, dims):
+                assert A0.shape == (n * p, n * p)
+                if n % (8 if form == 9 else 4) == 0:
+                    ctx.set_tuning(n, 0, form)
+                    assert ctx.get_tuning(n)[1] == form
+                devs.append(to_dev(A0))
+                taus.append(torch.zeros(n * p, dtype=torch.float64, device="cuda"))
+            torch.cuda.synchronize()
+            print(f"scan form {form} phases {phases}: " + ", ".join(f"n={n} p={p}" for n, p in dims), flush=True)
+            assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)
+                                          for d, t, (n, p) in zip(devs, taus, dims)],
+                                    phases=phases) == [0] * len(dims)
+            out[(form, phases)] = [from_dev(d, n * p) for d, (n, p) in zip(devs, dims)]
+            ctx.close()
+    for phases in (B.PHASE_ALL, diag_only):
+        for (n, p), a, b in zip(dims, out[(9, phases)], out[(10, phases)]):
+            if n % 8 == 0:
+                assert np.array_equal(a, b), (n, p, phases)
+    for A0, (n, p), G, D in zip(cases, dims, out[(10, B.PHASE_ALL)], out[(10, diag_only)]):
+        R = A0.copy(order="F")
+        O.invert(n, p, R)
+        assert rel(G, R) <= 1e-10, (n, p)
+        for l in range(p):
+            s = slice(l * n, (l + 1) * n)
+            assert np.array_equal(D[s, s], G[s, s]), (n, p, l)
 
 
 def test_det_matches_slogdet(B):
@@ -438,7 +490,7 @@ def test_diag_only_blocks_equal_full_inverse(B):
 
     cases = [(3, 5), (6, 4), (10, 3), (4, 7), (8, 3), (12, 3)]
     ctx = B.Context(0)
-    for mti in (0, 1, 2, 4, 9):
+    for mti in (0, 1, 2, 4, 9, 10):
         mats = []
         for k, (lx, p) in enumerate(cases):
             n = lx * lx
