@@ -1377,6 +1377,48 @@ __global__ __launch_bounds__(64 * NW) void bsoi_scan_kernel(const ScanTask* __re
 // ---------------------------------------------------------------------------
 // BSOI row-panel scan on v_mfma_f64_4x4x4f64 (K5, 4x4x4 form).
 // ---------------------------------------------------------------------------
+// Software pipeline of a k-loop over DEPTH register buffers (the 4x4x4
+// scan): step i uses buffer i mod DEPTH, whose operands were loaded
+// DEPTH - 1 steps earlier.  ld(u, step) loads buffer u, st(u) runs it.
+// pipe_prologue loads steps 0 .. DEPTH - 2 (those below S); pipe_steps runs
+// steps s + I .. s + R - 1 from a loop head at s (a multiple of DEPTH), each
+// first loading step s + i + DEPTH - 1 when it is below S (always in the
+// steady loop, STEADY = true skips the test); pipe_tail runs the R = S - s
+// last steps, 2 <= R <= 2 DEPTH - 2 (R < DEPTH - 1 only when the loop never
+// ran: all S >= 2 steps were loaded by the prologue), one unrolled sequence
+// each.
+template <int I, int DEPTH, class Ld>
+__device__ __forceinline__ void pipe_prologue(Ld& ld, int S) {
+  if constexpr (I < DEPTH - 1) {
+    if (I < 2 || I < S) ld(I, I);  // S >= 2 (n >= 8)
+    pipe_prologue<I + 1, DEPTH>(ld, S);
+  }
+}
+
+template <int I, int R, int DEPTH, bool STEADY, class Ld, class St>
+__device__ __forceinline__ void pipe_steps(Ld& ld, St& st, int s, int S) {
+  if constexpr (I < R) {
+    if constexpr (STEADY) {
+      ld((I + DEPTH - 1) % DEPTH, s + I + DEPTH - 1);
+    } else if constexpr (I + DEPTH - 1 < R) {
+      ld((I + DEPTH - 1) % DEPTH, s + I + DEPTH - 1);
+    }
+    st(I % DEPTH);
+    pipe_steps<I + 1, R, DEPTH, STEADY>(ld, st, s, S);
+  }
+}
+
+template <int R, int DEPTH, class Ld, class St>
+__device__ __forceinline__ void pipe_tail(Ld& ld, St& st, int s, int rem) {
+  if constexpr (R <= 2 * DEPTH - 2) {
+    if (rem == R) {
+      pipe_steps<0, R, DEPTH, false>(ld, st, s, s + R);
+      return;
+    }
+    pipe_tail<R + 1, DEPTH>(ld, st, s, rem);
+  }
+}
+
 // The same scan as bsoi_scan_kernel for a 32-row panel, on the 4-block
 // 4x4x4 FP64 MFMA (measured here at 1.5x the FP64 rate of the 16x16x4
 // form).  For lane l = 16 x + 4 blk + y the instruction takes
@@ -1397,6 +1439,8 @@ __global__ __launch_bounds__(64 * NW) void bsoi_scan_kernel(const ScanTask* __re
 // k-step's operands one step ahead, 3 two steps ahead (then every lane's
 // Q_k^T loads address one uniform row base plus a fixed per-lane offset).
 // Both run the same MFMAs in the same order, so the sums are the same.
+// (Four buffers, three steps ahead, measured the same as three at n = 100
+// and 144, where the registers allow them.)
 template <int NTW, int NW, int DEPTH = 2>
 __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* __restrict__ tasks,
                                                                 const Item* __restrict__ items, int ldcar) {
@@ -1514,37 +1558,16 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
         step(0, masked_tag);
         step(1, masked_tag);
       } else {
-        // S = n / 4 >= 2 k-steps; step s uses buffer s mod 3, loaded two
-        // steps ahead.  At the loop head buffers 0 and 1 hold steps s, s + 1.
+        // S = n / 4 >= 2 k-steps; step s uses buffer s mod DEPTH, loaded
+        // DEPTH - 1 steps ahead.  At the loop head buffers 0 .. DEPTH - 2
+        // hold steps s .. s + DEPTH - 2 (s a multiple of DEPTH).
         const int S = n >> 2;
-        load(0, 0);
-        load(1, 4);
+        auto st = [&](int u) { step(u, masked_tag); };
+        auto ld = [&](int u, int sx) { load(u, 4 * sx); };
+        pipe_prologue<0, DEPTH>(ld, S);
         int s = 0;
-        for (; s + 5 <= S; s += 3) {
-          load(2, 4 * s + 8);
-          step(0, masked_tag);
-          load(0, 4 * s + 12);
-          step(1, masked_tag);
-          load(1, 4 * s + 16);
-          step(2, masked_tag);
-        }
-        const int rem = S - s;  // 2, 3 or 4
-        if (rem == 2) {
-          step(0, masked_tag);
-          step(1, masked_tag);
-        } else if (rem == 3) {
-          load(2, 4 * s + 8);
-          step(0, masked_tag);
-          step(1, masked_tag);
-          step(2, masked_tag);
-        } else {
-          load(2, 4 * s + 8);
-          step(0, masked_tag);
-          load(0, 4 * s + 12);
-          step(1, masked_tag);
-          step(2, masked_tag);
-          step(0, masked_tag);
-        }
+        for (; s + 2 * DEPTH - 1 <= S; s += DEPTH) pipe_steps<0, DEPTH, DEPTH, true>(ld, st, s, S);
+        pipe_tail<2, DEPTH>(ld, st, s, S - s);  // S - s >= 2
       }
     };
     // Operand b of the 4-row band of Q_k^T that starts at row k0 of Q (Q_k
@@ -1606,14 +1629,15 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
           },
           std::false_type{});
     }
-    __syncthreads();
     // epilogue: lane holds rows 4 g + x, column tile * 16 + l16.  First the
-    // G block (columns n..2n-1 of the product), then X_{r,k-1} into Xs when
-    // block k - 1 has an X part (wave w: columns w, w + 4, ...), then the
-    // new carry, so that the stores' and the staging loads' latencies pass
-    // under the carry writes; the barrier waits for both.  (Mixed into the
-    // k-loop, a direct-to-LDS load among the loads of Q_k^T makes the
-    // compiler drain every load at each k-step.)
+    // G block (columns n..2n-1 of the product), which touches no LDS, so a
+    // wave stores it as soon as its own k-loop ends; then, once every wave is
+    // done with Xs and the carry, X_{r,k-1} into Xs when block k - 1 has an
+    // X part (wave w: columns w, w + 4, ...), then the new carry, so that
+    // the stores' and the staging loads' latencies pass under the carry
+    // writes; the last barrier waits for both.  (Mixed into the k-loop, a
+    // direct-to-LDS load among the loads of Q_k^T makes the compiler drain
+    // every load at each k-step.)
 #pragma unroll
     for (int b = 0; b < NTW; ++b) {
       const int col = (wave + NW * b) * 16 + l16;
@@ -1624,6 +1648,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
         if (row < N && (!diag || row / n == k + 1)) A[(size_t)((k + 1) * n + (col - n)) * lda + row] = acc[g][b];
       }
     }
+    __syncthreads();
     const bool stage = k - 1 >= kstop && rbmin <= k - 1;
     if (stage)
       for (int kk = wave; kk < n; kk += NW) stage_col(k - 1, kk);

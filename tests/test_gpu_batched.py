@@ -386,16 +386,15 @@ def test_tuned_variants_match_oracle(B, eps):
     ctx.close()
 
 
-def test_scan4_three_buffer_form_same_bits(B):
-    """The 4x4x4 scan with operands loaded two k-steps ahead (scan_mti 10)
-    runs the same MFMAs in the same order as the two-buffer form (9): the
-    same bits, full and diagonal-only, for block sizes whose 32-row panels
+def test_scan4_buffer_depths_same_bits(B):
+    """The 4x4x4 scan with operands loaded one (scan_mti 9) or two (10)
+    k-steps ahead runs the same MFMAs in the same order: the same bits, full and diagonal-only, for block sizes whose 32-row panels
     straddle block rows (n = 16, 24, 144) and whose k-step counts S = n / 4
-    give every tail of the three-buffer loop (S - 2 mod 3 = 0: n = 8, 32;
-    1: n = 24, 144; 2: n = 16, 64, 256), in one batch.  Form 10 alone also
-    takes n = 4 mod 8 (n = 20, 36, 100: a half column tile at the end of
-    Q_k^T), against the oracle at 1e-10 and with its diagonal-only blocks
-    equal to the full inverse's."""
+    reach every tail of the deeper loops (S = 2 .. 64), in one batch beside
+    a matrix on the 16x16x4 scan.  Form 10 also takes n = 4 mod 8 (n = 12,
+    20, 28, 36, 100: a half column tile at the end of Q_k^T); every result of
+    it matches the oracle at 1e-10 and has its diagonal-only blocks equal to
+    the full inverse's."""
     import torch
 
     O = oracle.Oracle("auto", threads=1)
@@ -403,8 +402,15 @@ def test_scan4_three_buffer_form_same_bits(B):
              dqmc.dqmc_matrix(12, 3, instance=122)[0], dqmc.dqmc_matrix(16, 2, instance=123)[0],
              dqmc.dqmc_matrix(10, 3, instance=124)[0], dqmc.random_pcyclic(8, 5, seed=125),
              dqmc.random_pcyclic(32, 4, seed=126), dqmc.random_pcyclic(24, 3, seed=127),
-             dqmc.dqmc_matrix(6, 6, instance=128)[0], dqmc.random_pcyclic(20, 4, seed=129)]
-    dims = [(16, 7), (64, 4), (144, 3), (256, 2), (100, 3), (8, 5), (32, 4), (24, 3), (36, 6), (20, 4)]
+             dqmc.dqmc_matrix(6, 6, instance=128)[0], dqmc.random_pcyclic(20, 4, seed=129),
+             dqmc.random_pcyclic(12, 5, seed=130), dqmc.random_pcyclic(40, 3, seed=131),
+             dqmc.random_pcyclic(28, 3, seed=132)]
+    dims = [(16, 7), (64, 4), (144, 3), (256, 2), (100, 3), (8, 5), (32, 4), (24, 3), (36, 6), (20, 4),
+            (12, 5), (40, 3), (28, 3)]
+
+    def takes(form, n):
+        return n % 8 == 0 if form == 9 else n % 4 == 0
+
     diag_only = B.PHASE_ALL | B.PHASE_DIAG_ONLY
     out = {}
     for form in (9, 10):
@@ -413,7 +419,7 @@ def test_scan4_three_buffer_form_same_bits(B):
             devs, taus = [], []
             for A0, (n, p) in zip(cases, dims):
                 assert A0.shape == (n * p, n * p)
-                if n % (8 if form == 9 else 4) == 0:
+                if takes(form, n):
                     ctx.set_tuning(n, 0, form)
                     assert ctx.get_tuning(n)[1] == form
                 devs.append(to_dev(A0))
@@ -426,16 +432,19 @@ def test_scan4_three_buffer_form_same_bits(B):
             out[(form, phases)] = [from_dev(d, n * p) for d, (n, p) in zip(devs, dims)]
             ctx.close()
     for phases in (B.PHASE_ALL, diag_only):
-        for (n, p), a, b in zip(dims, out[(9, phases)], out[(10, phases)]):
-            if n % 8 == 0:
-                assert np.array_equal(a, b), (n, p, phases)
-    for A0, (n, p), G, D in zip(cases, dims, out[(10, B.PHASE_ALL)], out[(10, diag_only)]):
-        R = A0.copy(order="F")
-        O.invert(n, p, R)
-        assert rel(G, R) <= 1e-10, (n, p)
-        for l in range(p):
-            s = slice(l * n, (l + 1) * n)
-            assert np.array_equal(D[s, s], G[s, s]), (n, p, l)
+        for i, (n, p) in enumerate(dims):
+            if takes(9, n):
+                assert np.array_equal(out[(9, phases)][i], out[(10, phases)][i]), (n, p, phases)
+    for form in (10,):
+        for A0, (n, p), G, D in zip(cases, dims, out[(form, B.PHASE_ALL)], out[(form, diag_only)]):
+            if not takes(form, n):
+                continue
+            R = A0.copy(order="F")
+            O.invert(n, p, R)
+            assert rel(G, R) <= 1e-10, (form, n, p)
+            for l in range(p):
+                s = slice(l * n, (l + 1) * n)
+                assert np.array_equal(D[s, s], G[s, s]), (form, n, p, l)
 
 
 def test_det_matches_slogdet(B):
