@@ -131,8 +131,9 @@ class DeviceSet:
 def scan_kernel_name(n, scan_mti=0):
     """The BSOI scan kernel a plan on block size n runs (the library's
     default scan form unless scan_mti overrides it)."""
-    form = scan_mti or (10 if 128 <= n <= 256 and n % 4 == 0 else (2 if n <= 256 else 1))
-    return ("bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form in (9, 10)
+    by4 = n % 4 == 0 and n >= 8
+    form = scan_mti or (10 if 128 <= n <= 256 and by4 else (12 if n <= 64 and by4 else (2 if n <= 256 else 1)))
+    return ("bsoi_scan4_kernel (v_mfma_f64_4x4x4f64)" if form in (9, 10, 12)
             else "bsoi_scan_kernel (v_mfma_f64_16x16x4f64)")
 
 
@@ -251,7 +252,9 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn, save_tune=None):
     """The uneven mixed workload: `count` matrices with lattice side from
     {6, 8, 10, 12, 16}, p from U{10..80}, U from U[2, 6] (seeded draw).
     (1) the autotuner picks nb, scan form, GEMM tile and reflector slabs per
-        workgroup per n, on trial batches of the workload's own matrices;
+        workgroup per n, on trial batches of the workload's own matrices,
+        then keeps each n's choice only where it beats the defaults in the
+        whole plan;
     (2) all matrices in ONE plan on the tuned context, captured in a HIP
         graph, `reps` executions on rebuilt inputs;
     (3) the plain path on the same matrices: one single-matrix plan each on
@@ -269,11 +272,16 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn, save_tune=None):
     # (a smaller batch makes different launches and picks different forms)
     table = autotune.tune(B, ctx, sorted({sp[0] for sp in specs}), p=tune_p, batch=len(specs), reps=1,
                           log=log_fn, specs=specs)
-    if save_tune:
-        autotune.save(table, save_tune)
     tune_s = time.perf_counter() - t0
     log_fn(f"[bench] uneven: building {count} matrices")
     mats = workload.DeviceMatrices(B, ctx, specs, seed=seed)
+    # each tuned block size against the defaults, in the whole plan
+    t0 = time.perf_counter()
+    autotune.refine(B, ctx, table, mats.descs(), mats.rebuild, reps=2, log=log_fn)
+    tune_s += time.perf_counter() - t0
+    mats.rebuild()
+    if save_tune:
+        autotune.save(table, save_tune)
     tot_flops = sum(flops.bsofi(n, p) for (_, n, p, _, _) in mats.specs)
     stream = torch.cuda.ExternalStream(ctx.stream)
     plan = B.Plan(ctx, mats.descs())
@@ -645,7 +653,7 @@ def main():
             "plain_resid_ok_sampled": u["plain_resid_ok"],
             "resid_max_over_bound": u["resid_max_rel"],
             "autotune": {str(k): f"nb={v['nb']} scan_mti={v['scan_mti']} gemm_tile={v['gemm_tile']} "
-                                 f"refl_spw={v['refl_spw']}" for k, v in u["tune"].items()},
+                                 f"refl_spw={v['refl_spw']} ({v.get('kept', 'tuned')})" for k, v in u["tune"].items()},
             "autotune_s": round(u["tune_s"], 1),
         }
     if args.serve:

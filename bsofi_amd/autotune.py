@@ -33,12 +33,14 @@ from .workload import DeviceMatrices
 def candidates(n):
     nbs = [nb for nb, ok in ((32, 2 * n <= 512), (16, True), (8, True)) if ok][:2]
     # scan forms: 16 * m rows on the 16x16x4 MFMA with 4 waves (m = 1, 2, 4),
-    # 8: 16 rows with 8 waves (n > 512), or 10: 32 rows on the 4x4x4 MFMA
-    # with operands loaded two k-steps ahead (4 | n).  Form 9, the same scan
-    # with two buffers (8 | n), gives the same bits and ran slower at every
-    # size measured, so the tuner leaves it out.
+    # 8: 16 rows with 8 waves (n > 512), 10: 32 rows on the 4x4x4 MFMA with
+    # operands loaded two k-steps ahead (4 | n), 12: the same with two
+    # workgroups per CU (n <= 64).  Form 9, the 4x4x4 scan with two buffers
+    # (8 | n), gives the same bits as 10 and ran slower at every size
+    # measured, so the tuner leaves it out.
+    by4 = n % 4 == 0 and n >= 8
     mtis = [m for m, ok in ((1, n <= 512), (2, n <= 256), (4, n <= 128), (8, n > 512),
-                            (10, n <= 256 and n % 4 == 0 and n >= 8)) if ok]
+                            (10, n <= 256 and by4), (12, n <= 64 and by4)) if ok]
     return [(nb, m) for nb in nbs for m in mtis]
 
 
@@ -121,6 +123,47 @@ def tune(B, ctx, sides, p=24, batch=8, reps=3, log=None, specs=None, margin=0.03
                     "trial": [list(t[:2]) for t in trial],
                     "all": {",".join(str(v) for v in k): round(v, 4) for k, v in res.items()}}
         mats.release()
+    return table
+
+
+def refine(B, ctx, table, descs, rebuild, reps=2, log=None, margin=0.005):
+    """Second pass on the whole workload (`descs`, already on the device,
+    `rebuild` restoring their inputs).  Block sizes tuned in isolation share
+    the GPU in the real plan, so for each tuned n (largest first) the whole
+    plan is timed with up to three rows for that n: the tuned one, the
+    library's defaults, and the fastest trial row with another scan form;
+    the defaults stay unless another row is faster by more than `margin`.
+    Updates `table` in place (row["kept"]: "tuned", "default" or
+    "alternative") and the context's tuning."""
+    for n in sorted(table, key=int, reverse=True):
+        row = table[n]
+        tuned = (int(row["nb"]), int(row["scan_mti"]), int(row.get("gemm_tile", 0)), int(row.get("refl_spw", 0)))
+        ctx.set_tuning(int(n), 0, 0, 0, 0)
+        default = ctx.get_tuning_ex(int(n))
+        cands = {"default": default}
+        if tuned != default:
+            cands["tuned"] = tuned
+        forms = {default[1], tuned[1]}
+        for key, _ in sorted(row.get("all", {}).items(), key=lambda kv: kv[1]):
+            k = tuple(int(v) for v in key.split(","))
+            if k[1] not in forms:
+                cands["alternative"] = k
+                break
+        times = {}
+        for name, k in cands.items():
+            ctx.set_tuning(int(n), *k)
+            times[name] = time_plan(B, ctx, descs, reps, rebuild)
+            rebuild()
+        best = min(times, key=times.get)
+        if times[best] >= (1.0 - margin) * times["default"]:
+            best = "default"
+        k = cands[best]
+        ctx.set_tuning(int(n), *k)
+        row.update(nb=k[0], scan_mti=k[1], gemm_tile=k[2], refl_spw=k[3], kept=best,
+                   whole_ms={name: round(t, 3) for name, t in times.items()})
+        if log:
+            log(f"[tune] whole workload, n={n}: " + ", ".join(f"{name} {cands[name]} {t:.2f} ms"
+                                                             for name, t in times.items()) + f" -> {best}")
     return table
 
 

@@ -104,11 +104,14 @@ static bool nb_supported(int n, int nb) {
 // on 16 * scan_mti rows; kScan8W is the same kernel on 16 rows with 8 waves
 // (block sizes 512 < n <= 1024: 16 output tiles per wave instead of 32);
 // kScan4 is bsoi_scan4_kernel (4x4x4 MFMA) on 32 rows, kScan4D the same
-// kernel with its operands loaded two k-steps ahead (three buffers).
+// kernel with its operands loaded two k-steps ahead (three buffers), kScan4S
+// that form with two workgroups per CU (n <= 64: a quarter of the LDS
+// staging, and registers for two waves per SIMD).
 static constexpr int kScan8W = 8;
 static constexpr int kScan4 = 9;
 static constexpr int kScan4D = 10;
-static bool scan_is4(int mti) { return mti == kScan4 || mti == kScan4D; }
+static constexpr int kScan4S = 12;
+static bool scan_is4(int mti) { return mti == kScan4 || mti == kScan4D || mti == kScan4S; }
 static int scan_rows(int mti) { return scan_is4(mti) ? 32 : (mti == kScan8W ? 16 : 16 * mti); }
 static int scan_waves(int mti) { return mti == kScan8W ? 8 : 4; }
 static int scan_ntw_for(int n, int mti) {
@@ -119,6 +122,7 @@ static int scan_ntw_for(int n, int mti) {
 static bool scan_variant_ok(int mti, int ntw) {
   // with 8 | n, or 4 | n for kScan4D (scan_ok); ntw <= 8 means n <= 256, the column count of the
   // kernel's LDS staging buffer for X_{r,k}
+  if (mti == kScan4S) return ntw >= 1 && ntw <= 2;
   if (scan_is4(mti)) return ntw >= 1 && ntw <= 8;
   if (mti == kScan8W) return ntw >= 9 && ntw <= 16;
   if (mti == 1) return ntw >= 1 && ntw <= 16;
@@ -131,7 +135,8 @@ static bool scan_variant_ok(int mti, int ntw) {
 // The two-buffer loop runs n / 4 k-steps in pairs (8 | n), the three-buffer
 // loop any n / 4 >= 2 (4 | n).
 static bool scan_ok(int n, int mti, int ntw) {
-  return scan_variant_ok(mti, ntw) && (mti != kScan4 || n % 8 == 0) && (mti != kScan4D || (n % 4 == 0 && n >= 8));
+  return scan_variant_ok(mti, ntw) && (mti != kScan4 || n % 8 == 0) &&
+         (mti == kScan4 || !scan_is4(mti) || (n % 4 == 0 && n >= 8));
 }
 
 static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
@@ -156,11 +161,15 @@ static Geo geometry(int n, int p, int lda, const Tune* tune = nullptr) {
   const int nw = p8 ? 8 : 4;
   g.panel_variant = nw * 10000 + rpg * 100 + g.nb / nw;
   // default scan form: the three-buffer 4x4x4 scan from n = 128 (one
-  // workgroup per CU; below that its k-loops are short and batches of small
-  // matrices need the occupancy of the 16x16x4 form); 8 waves beyond n = 512
+  // workgroup per CU) and up to n = 64 (two per CU), for 4 | n; in between,
+  // and for other n, the 16x16x4 form (batches of n = 100 ran faster on it,
+  // a single matrix on the 4x4x4 form: the autotuner decides); 8 waves
+  // beyond n = 512
+  const bool by4 = n % 4 == 0 && n >= 8;
   g.scan_mti = (tune && tune->scan_mti)
                    ? tune->scan_mti
-                   : (n <= 256 ? (n >= 128 && n % 4 == 0 ? kScan4D : 2) : (n <= 512 ? 1 : kScan8W));
+                   : (n <= 256 ? (by4 && n >= 128 ? kScan4D : (by4 && n <= 64 ? kScan4S : 2))
+                               : (n <= 512 ? 1 : kScan8W));
   g.scan_ntw = scan_ntw_for(n, g.scan_mti);
   g.gemm_tile = tune ? tune->gemm_tile : 0;
   g.refl_spw = tune ? tune->refl_spw : 0;
@@ -940,17 +949,17 @@ static void set_scan_attr(size_t shmem) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
 }
 
-template <int W, int D>
+template <int W, int D, int XN = 256, int MINB = 1>
 static void launch_scan4(const Launch& L, const char* base, hipStream_t s) {
-  hipLaunchKernelGGL((bsoi_scan4_kernel<W, 4, D>), dim3(L.grid), dim3(256), L.shmem, s,
+  hipLaunchKernelGGL((bsoi_scan4_kernel<W, 4, D, XN, MINB>), dim3(L.grid), dim3(256), L.shmem, s,
                      (const ScanTask*)(base + L.task_off), (const Item*)(base + L.item_off), L.ldcar);
 }
 
-template <int W, int D>
+template <int W, int D, int XN = 256, int MINB = 1>
 static void set_scan_attr4(size_t shmem) {
   if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)bsoi_scan4_kernel<W, 4, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)shmem);
+    (void)hipFuncSetAttribute((const void*)bsoi_scan4_kernel<W, 4, D, XN, MINB>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
 }
 
 // scan instances: variant = MTI * 100 + NTW (see scan_variant_ok); the
@@ -960,11 +969,14 @@ static void set_scan_attr4(size_t shmem) {
 #define SCAN4_C(FN, W, ...)                                  \
   case kScan4 * 100 + W: FN##4<W, 2>(__VA_ARGS__); break; \
   case kScan4D * 100 + W: FN##4<W, 3>(__VA_ARGS__); break;
+#define SCAN4S_C(FN, W, ...) \
+  case kScan4S * 100 + W: FN##4<W, 3, 64, 2>(__VA_ARGS__); break;
 #define SCAN_SWITCH(FN, ...)                                                                                 \
   switch (variant) {                                                                                         \
     SCAN4_C(FN, 1, __VA_ARGS__) SCAN4_C(FN, 2, __VA_ARGS__) SCAN4_C(FN, 3, __VA_ARGS__)                      \
     SCAN4_C(FN, 4, __VA_ARGS__) SCAN4_C(FN, 5, __VA_ARGS__) SCAN4_C(FN, 6, __VA_ARGS__)                      \
     SCAN4_C(FN, 7, __VA_ARGS__) SCAN4_C(FN, 8, __VA_ARGS__)                                                  \
+    SCAN4S_C(FN, 1, __VA_ARGS__) SCAN4S_C(FN, 2, __VA_ARGS__)                                                \
     SCAN_C(FN, 1, 1, __VA_ARGS__) SCAN_C(FN, 1, 2, __VA_ARGS__) SCAN_C(FN, 1, 3, __VA_ARGS__)                \
     SCAN_C(FN, 1, 4, __VA_ARGS__) SCAN_C(FN, 1, 5, __VA_ARGS__) SCAN_C(FN, 1, 6, __VA_ARGS__)                \
     SCAN_C(FN, 1, 7, __VA_ARGS__) SCAN_C(FN, 1, 8, __VA_ARGS__) SCAN_C(FN, 1, 9, __VA_ARGS__)                \

@@ -1440,10 +1440,14 @@ __device__ __forceinline__ void pipe_tail(Ld& ld, St& st, int s, int rem) {
 // Q_k^T loads address one uniform row base plus a fixed per-lane offset).
 // Both run the same MFMAs in the same order, so the sums are the same.
 // (Four buffers, three steps ahead, measured the same as three at n = 100
-// and 144, where the registers allow them.)
-template <int NTW, int NW, int DEPTH = 2>
-__global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* __restrict__ tasks,
-                                                                const Item* __restrict__ items, int ldcar) {
+// and 144, where the registers allow them.)  XN is the largest n the X
+// staging buffer holds (256), or 64 for the form that runs two workgroups
+// per CU (MINB = 2: a quarter of the LDS, at most 256 registers per lane):
+// batches of small matrices, whose k-loops are too short for one wave per
+// SIMD to keep the matrix cores busy on its own.
+template <int NTW, int NW, int DEPTH = 2, int XN = 256, int MINB = 1>
+__global__ __launch_bounds__(64 * NW, MINB) void bsoi_scan4_kernel(const ScanTask* __restrict__ tasks,
+                                                                   const Item* __restrict__ items, int ldcar) {
   static_assert(DEPTH == 2 || DEPTH == 3, "two or three operand buffers");
   constexpr int NT = 64 * NW;
   constexpr int MT = 32;
@@ -1451,9 +1455,9 @@ __global__ __launch_bounds__(64 * NW, 1) void bsoi_scan4_kernel(const ScanTask* 
   extern __shared__ double carry[];  // carry[kk * ldcar + (i & 3) * 8 + (i >> 2)] = row i, column kk
   // X_{r,k} (the panel's 32 rows of block column k of A), staged by direct
   // global -> LDS loads during block k + 1's epilogue: Xs[kk * XLD + i].
-  // The planner takes this form for n <= 256 only (NTW <= 8).
-  __shared__ double Xs[256 * XLD];
-  static_assert(NTW <= 8, "n <= 256: X_{r,k} must fit Xs");
+  // The planner takes this form for n <= XN only (NTW <= XN / 32).
+  __shared__ double Xs[XN * XLD];
+  static_assert(NTW * 32 <= XN, "n <= XN: X_{r,k} must fit Xs");
   const Item it = items[blockIdx.x];
   const ScanTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;

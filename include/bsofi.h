@@ -177,7 +177,9 @@ int bsofi_ctx_synchronize(bsofi_ctx_t ctx);
  * 8 waves (512 < n <= 1024, the default there), 9 = 32 rows on the 4x4x4
  * FP64 MFMA with two operand buffers (needs n <= 256 and n % 8 == 0), 10 =
  * the same with three buffers, loading two k-steps ahead (n <= 256,
- * n % 4 == 0; the default for n >= 128; the same bits as 9).  0
+ * n % 4 == 0; the default for n >= 128; the same bits as 9), 12 = form 10
+ * with two workgroups per CU (8 <= n <= 64, n % 4 == 0; the default there;
+ * the same bits as 10).  0
  * keeps the default; unsupported pairs return BSOFI_ERR_UNSUPPORTED.  No
  * counterpart in the reference, whose static CPU/GPU split (KAPPA_*,
  * *_SWITCH_*) these choices replace. */

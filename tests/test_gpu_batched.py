@@ -381,20 +381,21 @@ def test_tuned_variants_match_oracle(B, eps):
     with pytest.raises(ValueError):
         ctx.set_tuning(144, 0, 4)  # 64-row scan tiles need n <= 128
     ctx.clear_tuning()
-    assert ctx.get_tuning(64) == (32, 2)
+    assert ctx.get_tuning(64) == (32, 12)  # the 4x4x4 scan, two workgroups per CU, for 4 | n <= 64
     assert ctx.get_tuning(144) == (32, 10)  # the three-buffer 4x4x4 scan for 4 | n in [128, 256]
     ctx.close()
 
 
 def test_scan4_buffer_depths_same_bits(B):
     """The 4x4x4 scan with operands loaded one (scan_mti 9) or two (10)
-    k-steps ahead runs the same MFMAs in the same order: the same bits, full and diagonal-only, for block sizes whose 32-row panels
+    k-steps ahead, and the latter with two workgroups per CU (12, n <= 64),
+    run the same MFMAs in the same order: the same bits, full and diagonal-only, for block sizes whose 32-row panels
     straddle block rows (n = 16, 24, 144) and whose k-step counts S = n / 4
     reach every tail of the deeper loops (S = 2 .. 64), in one batch beside
-    a matrix on the 16x16x4 scan.  Form 10 also takes n = 4 mod 8 (n = 12,
-    20, 28, 36, 100: a half column tile at the end of Q_k^T); every result of
-    it matches the oracle at 1e-10 and has its diagonal-only blocks equal to
-    the full inverse's."""
+    a matrix on the 16x16x4 scan.  Forms 10 and 12 also take n = 4 mod 8
+    (n = 12, 20, 28, 36, 100: a half column tile at the end of Q_k^T); every
+    result of theirs matches the oracle at 1e-10 and has its diagonal-only
+    blocks equal to the full inverse's."""
     import torch
 
     O = oracle.Oracle("auto", threads=1)
@@ -409,11 +410,11 @@ def test_scan4_buffer_depths_same_bits(B):
             (12, 5), (40, 3), (28, 3)]
 
     def takes(form, n):
-        return n % 8 == 0 if form == 9 else n % 4 == 0
+        return n % 8 == 0 if form == 9 else (n % 4 == 0 and (form == 10 or n <= 64))
 
     diag_only = B.PHASE_ALL | B.PHASE_DIAG_ONLY
     out = {}
-    for form in (9, 10):
+    for form in (9, 10, 12):
         for phases in (B.PHASE_ALL, diag_only):
             ctx = B.Context(0)
             devs, taus = [], []
@@ -433,9 +434,10 @@ def test_scan4_buffer_depths_same_bits(B):
             ctx.close()
     for phases in (B.PHASE_ALL, diag_only):
         for i, (n, p) in enumerate(dims):
-            if takes(9, n):
-                assert np.array_equal(out[(9, phases)][i], out[(10, phases)][i]), (n, p, phases)
-    for form in (10,):
+            for f1, f2 in ((9, 10), (10, 12)):
+                if takes(f1, n) and takes(f2, n):
+                    assert np.array_equal(out[(f1, phases)][i], out[(f2, phases)][i]), (n, p, f1, f2, phases)
+    for form in (10, 12):
         for A0, (n, p), G, D in zip(cases, dims, out[(form, B.PHASE_ALL)], out[(form, diag_only)]):
             if not takes(form, n):
                 continue
@@ -499,7 +501,7 @@ def test_diag_only_blocks_equal_full_inverse(B):
 
     cases = [(3, 5), (6, 4), (10, 3), (4, 7), (8, 3), (12, 3)]
     ctx = B.Context(0)
-    for mti in (0, 1, 2, 4, 9, 10):
+    for mti in (0, 1, 2, 4, 9, 10, 12):
         mats = []
         for k, (lx, p) in enumerate(cases):
             n = lx * lx

@@ -169,8 +169,9 @@ def test_c3_batch_512_matches_oracle_and_alone(B, O, eps):
 
 def test_c4_uneven_tuned_batch(B, O, eps):
     """The uneven workload: 256 matrices, lattice side in {6, 8, 10, 12, 16},
-    p in U{10..80}, U in U[2, 6], in ONE plan on an autotuned context,
-    replayed as a HIP graph.  Residual on every matrix; a sample with
+    p in U{10..80}, U in U[2, 6], in ONE plan on an autotuned context (per
+    block size on its own matrices, then each choice against the defaults
+    in the whole plan), replayed as a HIP graph.  Residual on every matrix; a sample with
     p <= 32 (beta <= 4) against the oracle at 1e-10; a sample over every n
     bit-equal to the same matrix alone on the same context."""
     from bsofi_amd import autotune, workload
@@ -181,6 +182,12 @@ def test_c4_uneven_tuned_batch(B, O, eps):
     print("case C4 tuned: " + ", ".join(f"n={k}: nb={v['nb']} scan_mti={v['scan_mti']}"
                                         for k, v in sorted(table.items())), flush=True)
     mats = workload.DeviceMatrices(B, ctx, specs)
+    # each tuned row against the defaults in the whole plan, then fresh inputs
+    autotune.refine(B, ctx, table, mats.descs(), mats.rebuild, reps=1)
+    mats.rebuild()
+    assert all(row["kept"] in ("tuned", "default", "alternative") for row in table.values())
+    print("  after the whole-plan pass: " + ", ".join(f"n={k}: scan_mti={v['scan_mti']} ({v['kept']})"
+                                                     for k, v in sorted(table.items())), flush=True)
     # per lattice side: one matrix with p <= 32 (oracle) and the one with the largest p (alone only)
     oracle_k, alone_k = [], []
     for side in sorted({sp[0] for sp in specs}):
