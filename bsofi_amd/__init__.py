@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -322,6 +323,10 @@ class Context:
             self.h = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -392,6 +397,10 @@ class Plan:
             self.h = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

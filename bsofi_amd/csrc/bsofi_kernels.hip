@@ -826,6 +826,12 @@ __global__ __launch_bounds__(512) void bsof_flow_kernel(const FlowTask* __restri
     __syncthreads();
     if (wave == 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // The fence's L2 write-back (buffer_wbl2) is counted in vmcnt, but the
+      // compiler drops the wait that should follow it when no other memory
+      // operation is outstanding; without this wait a flag could reach memory
+      // before the data it publishes and a workgroup on another XCD could read
+      // stale values.
+      __builtin_amdgcn_s_waitcnt(0x0F70);
       if (f.kind == 0) {
         if (lane == 0) st_agent(M.done + f.g, 1);
       } else {

@@ -11,6 +11,7 @@ percentiles.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import time
 
 import numpy as np
@@ -104,6 +105,10 @@ class Server:
             self.h = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be torn down:
+        # leave the memory to the process exit instead of calling into it
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
