@@ -84,6 +84,10 @@ struct Geo {
 
 static int nb_for(int n) { return (2 * n <= 512) ? 32 : 16; }
 
+// Panel variants: NW * 10000 + RPG * 100 + CPW for panel_factor_kernel,
+// plus kPanelCyc for panel_cyc_kernel (the 8-wave 32-wide instances).
+static constexpr int kPanelCyc = 100000;
+
 // Choices a context can override per block size n (the autotuner's knobs):
 // the sub-panel width nb of BSOF / BSOI and the rows per workgroup of the
 // BSOI scan (16 * scan_mti).  Zero keeps the default.
@@ -606,10 +610,10 @@ struct Planner {
   // column.  Rows beyond m enter only as zeros (fma(0, 0, s) = s) and are
   // never stored, so the factors do not depend on the instance.
   static int panel_instance(int variant, const std::vector<PanelTask>& t) {
-    if (variant != 80804) return variant;
+    if (variant != 80804 && variant != kPanelCyc + 80804) return variant;
     int mm = 0;
     for (const PanelTask& q : t) mm = std::max(mm, q.m);
-    return 80004 + 100 * std::max(2, (mm + 63) / 64);
+    return variant - 800 + 100 * std::max(2, (mm + 63) / 64);
   }
 
   // sign(det A) and log|det A| from R's diagonal and tau, before BSTRI
@@ -1005,7 +1009,13 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
   case W * 10000 + R * 100 + C:                                                                     \
     hipLaunchKernelGGL((panel_factor_kernel<R, C, W>), dim3(L.grid), dim3(64 * W), 0, s, t, it); \
     break;
+#define PANEL_CYC_CASE(R)                                                                                 \
+  case kPanelCyc + 80000 + R * 100 + 4:                                                                   \
+    hipLaunchKernelGGL((panel_cyc_kernel<R, 4, 8>), dim3(L.grid), dim3(512), 0, s, t, it);                \
+    break;
       switch (L.variant) {
+        PANEL_CYC_CASE(2) PANEL_CYC_CASE(3) PANEL_CYC_CASE(4) PANEL_CYC_CASE(5) PANEL_CYC_CASE(6)
+        PANEL_CYC_CASE(7) PANEL_CYC_CASE(8)
         PANEL_CASE(2, 4, 4) PANEL_CASE(4, 4, 4) PANEL_CASE(8, 4, 4) PANEL_CASE(16, 4, 4)
         PANEL_CASE(2, 2, 4) PANEL_CASE(4, 2, 4) PANEL_CASE(8, 2, 4) PANEL_CASE(16, 2, 4) PANEL_CASE(32, 2, 4)
         PANEL_CASE(2, 4, 8) PANEL_CASE(3, 4, 8) PANEL_CASE(4, 4, 8) PANEL_CASE(5, 4, 8) PANEL_CASE(6, 4, 8)
@@ -1013,6 +1023,7 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
         default: break;
       }
 #undef PANEL_CASE
+#undef PANEL_CYC_CASE
       break;
     }
     case F_REFL: {
@@ -1102,7 +1113,9 @@ static bool known_variant(const Launch& L) {
   switch (L.family) {
     case F_PANEL: {
       static const int ok[] = {40204, 40404, 40804, 41604, 40202, 40402, 40802, 41602, 43202, 80204, 80304,
-                               80404, 80504, 80604, 80704, 80804, 83202};
+                               80404, 80504, 80604, 80704, 80804, 83202, kPanelCyc + 80204,
+                               kPanelCyc + 80304, kPanelCyc + 80404, kPanelCyc + 80504, kPanelCyc + 80604,
+                               kPanelCyc + 80704, kPanelCyc + 80804};
       return std::find(std::begin(ok), std::end(ok), L.variant) != std::end(ok);
     }
     case F_REFL:
@@ -1357,6 +1370,9 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
   for (int i = 0; i < count; ++i) {
     MatJob& J = jobs[i];
     J.g = geometry(mats[i].n, mats[i].p, mats[i].lda, ctx_tune(ctx, mats[i].n));
+    // the 32-wide 8-wave sub-panels take the cyclic panel (panel_cyc_kernel,
+    // the bits of panel_factor_kernel without its per-column barrier)
+    if (J.g.panel_variant == 80804) J.g.panel_variant += kPanelCyc;
     J.A = mats[i].A;
     J.tau = mats[i].tau;
     J.info = P->d_info + i;

@@ -26,7 +26,9 @@
 #include <math.h>
 
 #include <type_traits>
+#include <utility>
 
+#include "bsofi.h"
 #include "bsofi_tasks.h"
 
 namespace bsofi {
@@ -180,6 +182,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return readlane64(v, 0);
 }
 
+// Workgroup barrier that orders LDS accesses only: global stores issued
+// before it stay in flight (__syncthreads also waits for them).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // T of the compact WY form (dlarft, forward, columnwise) from the Gram
 // entries G[j][c] = v_j^T v_c (read for j < c only) and tau (zero for the
 // columns at and beyond nbj), by all 256 threads of the workgroup.
@@ -191,7 +201,8 @@ __device__ __forceinline__ double wave_max(double v) {
 // column recurrence T[0:c, c] = -tau_c T[0:c, 0:c] V^T v_c of dlarft up to
 // rounding.  Tl and X are LDS scratch of NB x (NB + 1).  Must be called by
 // every thread of the block.  Writes the nbj x nbj factor, Tout[c * ldt + i]
-// for i, c < nbj (ldt >= nbj may be below NB).
+// for i, c < nbj (ldt >= nbj may be below NB).  Its barriers order LDS only,
+// so global stores issued before the call stay in flight.
 template <int NB, int NT = 256>
 __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double* taus, int nbj,
                                         gptr<double> Tout, int ldt, double (*Tl)[NB + 1],
@@ -202,7 +213,7 @@ __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double*
     const int i = e / NB, j = e % NB;
     Tl[i][j] = (i == j) ? taus[i] : 0.0;
   }
-  __syncthreads();
+  lds_sync();
 #pragma unroll
   for (int b = 1; b < NB; b *= 2) {
     const int e = tid;
@@ -216,14 +227,14 @@ __device__ __forceinline__ void build_T(const double (*G)[NB + 1], const double*
       for (int j = 0; j < b; ++j) acc = fma(G[s0 + r][s0 + b + j], Tl[s0 + b + j][s0 + b + cq], acc);
       X[s0 + r][s0 + b + cq] = acc;
     }
-    __syncthreads();
+    lds_sync();
     if (act) {  // T12 = -T1 X
       double acc = 0.0;
 #pragma unroll
       for (int j = 0; j < b; ++j) acc = fma(Tl[s0 + r][s0 + j], X[s0 + j][s0 + b + cq], acc);
       Tl[s0 + r][s0 + b + cq] = -acc;
     }
-    __syncthreads();
+    lds_sync();
   }
   for (int e = tid; e < nbj * nbj; e += NT) {
     const int c = e / nbj, i = e % nbj;
@@ -472,10 +483,9 @@ __device__ __forceinline__ void panel_body(const PanelTask& t) {
   };
   if (run(std::false_type{})) run(std::true_type{});
 
-  // G and taus are complete (the final barrier of run); T first, so that the
-  // barriers inside build_T do not come after the stores.
-  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
-  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Xl);
+  // G and taus are complete (the final barrier of run).  The panel's stores
+  // go out first and stay in flight under build_T, whose barriers order LDS
+  // only.
 #pragma unroll
   for (int q = 0; q < RPG; ++q) {
     const int row = lane + 64 * q;
@@ -485,12 +495,261 @@ __device__ __forceinline__ void panel_body(const PanelTask& t) {
       if (row < m && col < nbj) Pg[(size_t)col * lda + row] = a[q][jj];
     }
   }
+  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
+  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Xl);
 }
 
 template <int RPG, int CPW, int NW>
 __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* __restrict__ tasks,
                                                                const Item* __restrict__ items) {
   panel_body<RPG, CPW, NW>(tasks[items[blockIdx.x].task]);
+}
+
+// f(std::integral_constant<int, i>{}) for i = 0 .. N-1 in order.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Sub-panel Householder QR, cyclic form (K1c).  The same arithmetic as
+// panel_body -- the same FMA chains and the same butterfly reduction tree
+// (stages pair lanes 32, 16, ... apart, whichever lane keeps the sum; the
+// adds are commutative), so the same bits -- without a workgroup barrier per
+// column:
+//   * column c belongs to wave c mod NW (local column c / NW), so the owner
+//     of the next column is never the wave that has just formed one;
+//   * every v_c and tau_c stays in LDS (Vl, taus) and a counter `ready`
+//     (columns published) replaces the barrier: a wave waits for column c
+//     only when it needs v_c, and may lag the chain by any number of
+//     columns;
+//   * the owner of c + 1 applies H_c to that column first, with a
+//     reduction of that one dot product, and forms its reflector at raised
+//     wave priority; only then does it apply H_c to its other columns (one
+//     transposed reduction of all its dot products, which also yields the
+//     Gram entries v_j^T v_c of its finished columns j < c for T).
+// Scheduling barriers keep the compiler from moving the other columns'
+// work into the hand-off chain (without them it did, and the chain ran four
+// times longer).  Rows beyond m add exact zeros, so the bits do not depend
+// on the instance (RPG) or on the timing of the waves either.  A column
+// leaving the fast path's range makes the whole panel run again on LAPACK's
+// scaled path, as in panel_body.
+template <int RPG, int CPW, int NW>
+__device__ __forceinline__ void panel_cyc_body(const PanelTask& t) {
+  constexpr int NB = NW * CPW;
+  constexpr int NT = 64 * NW;
+  static_assert(NB < 64, "rows c and c + 1 of column c + 1 are lanes of the first row group");
+  static_assert(2 * NB * (NB + 1) <= NB * RPG * 64, "build_T's scratch fits in Vl");
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m = t.m, nbj = t.nbj, lda = t.lda;
+  const gptr<double> Pg = glb(t.P);
+
+  __shared__ double Vl[NB][RPG][64];  // v_c at rows lane + 64 q
+  __shared__ double G[NB][NB + 1];
+  __shared__ double taus[NB];
+  __shared__ int ready;
+  // polls of `ready` before a wave gives up on a column: far beyond any
+  // real wait (a whole sub-panel takes well under a million clocks), so
+  // reached only through a defect, which then ends in an error code, not a
+  // hung GPU
+  constexpr int kMaxPolls = 1 << 22;
+
+  double a[RPG][CPW];  // column wave + NW jj, rows lane + 64 q
+  auto run = [&](auto safe_tag) -> bool {
+    constexpr bool SAFE = decltype(safe_tag)::value;
+    bool bad = false, stuck = false;
+    for (int e = tid; e < NB * (NB + 1); e += NT) (&G[0][0])[e] = 0.0;
+    if (tid < NB) taus[tid] = 0.0;
+    if (tid == 0) ready = 0;
+#pragma unroll
+    for (int q = 0; q < RPG; ++q) {
+      const int row = lane + 64 * q;
+#pragma unroll
+      for (int jj = 0; jj < CPW; ++jj) {
+        const int col = wave + NW * jj;
+        a[q][jj] = (row < m && col < nbj) ? Pg[(size_t)col * lda + row] : 0.0;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), as in panel_body
+    __syncthreads();                     // G, taus and ready cleared before any publish
+
+    double vo[RPG];  // v and tau of the last reflector this wave formed
+    double tau_o = 0.0;
+    // Reflector of column c (local column jc of the calling wave) from
+    // alpha = a[c] and ||x||^2 over the rows below: v_c to Vl[c] and vo,
+    // tau_c to taus[c]; the column becomes beta on the diagonal and v below
+    // it; then ready = c + 1 (workgroup-scope release: the LDS writes are
+    // complete before any wave can see the count).
+    auto reflector = [&](int jc, int c, double alpha, double xn2) {
+      const bool inrange = xn2 > 0x1p-960 && xn2 < 0x1p960 && fabs(alpha) < 0x1p480;
+      double beta = -copysign(sqrt_normal(fma(alpha, alpha, xn2)), alpha);
+      const double amb = alpha - beta;
+      const double rr = rcp_normal(beta * amb);
+      double tau = -(amb * amb) * rr;
+      double scale = beta * rr;
+      if constexpr (SAFE) {
+        if (!inrange) {
+          double x[RPG];
+          x[0] = (lane > c) ? a[0][jc] : 0.0;
+#pragma unroll
+          for (int q = 1; q < RPG; ++q) x[q] = a[q][jc];
+          const Larfg lf = dlarfg_scalar(alpha, scaled_norm_wave<RPG>(x, lane));
+          tau = lf.tau;
+          beta = lf.beta;
+          scale = lf.scale;
+        }
+      } else {
+        bad |= !inrange;
+      }
+      const double v0 = (lane > c) ? a[0][jc] * scale : (lane == c ? 1.0 : 0.0);
+      Vl[c][0][lane] = v0;
+      vo[0] = v0;
+      a[0][jc] = (lane > c) ? v0 : (lane == c ? beta : a[0][jc]);
+#pragma unroll
+      for (int q = 1; q < RPG; ++q) {
+        const double v = a[q][jc] * scale;
+        Vl[c][q][lane] = v;
+        vo[q] = v;
+        a[q][jc] = v;
+      }
+      tau_o = tau;
+      if (lane == 0) taus[c] = tau;
+      __hip_atomic_store(&ready, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    // ||x||^2 of local column jc below row c, summed from the column
+    auto plain_norm2 = [&](int jc, int c) {
+      const double x0 = (lane > c) ? a[0][jc] : 0.0;
+      double n0 = fma(x0, x0, 0.0), n1 = 0.0;
+#pragma unroll
+      for (int q = 1; q < RPG; ++q) {
+        if (q & 1) n1 = fma(a[q][jc], a[q][jc], n1);
+        else n0 = fma(a[q][jc], a[q][jc], n0);
+      }
+      return wave_sum(n0 + n1, lane);
+    };
+
+    // A wave with no column below nbj has nothing to do in the loop.
+    if (wave < nbj) {
+      if (wave == 0) reflector(0, 0, readlane64(a[0][0], 0), plain_norm2(0, 0));
+      auto column_round = [&](auto jc_tag) {
+        constexpr int jc = decltype(jc_tag)::value;
+        for (int o = 0; o < NW; ++o) {
+          const int c = jc * NW + o;  // the column whose reflector is applied
+          if (c >= nbj) break;
+          const bool crit = c + 1 < nbj && wave == (o + 1) % NW;
+          double v[RPG];
+          double tau;
+          if (wave == o) {
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) v[q] = vo[q];
+            tau = tau_o;
+          } else {
+            int seen = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            for (int polls = 0; seen <= c; ++polls) {
+              if (polls >= kMaxPolls) {
+                stuck = true;
+                break;
+              }
+              if (!crit) __builtin_amdgcn_s_sleep(1);
+              seen = __builtin_amdgcn_readfirstlane(
+                  __hip_atomic_load(&ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+            for (int q = 0; q < RPG; ++q) v[q] = Vl[c][q][lane];
+            tau = taus[c];
+          }
+          // the next column, if this wave owns it: local index jc
+          // (o < NW - 1) or jc + 1 (o = NW - 1)
+          auto critical = [&](auto jn_tag) {
+            constexpr int jn = decltype(jn_tag)::value;
+            if constexpr (jn < CPW) {
+              __builtin_amdgcn_sched_barrier(0);
+              __builtin_amdgcn_s_setprio(3);
+              const int c1 = c + 1;
+              double s0 = fma(v[0], a[0][jn], 0.0), s1 = 0.0;
+#pragma unroll
+              for (int q = 1; q < RPG; ++q) {
+                if (q & 1) s1 = fma(v[q], a[q][jn], s1);
+                else s0 = fma(v[q], a[q][jn], s0);
+              }
+              const double w = tau * wave_sum(s0 + s1, lane);
+#pragma unroll
+              for (int q = 0; q < RPG; ++q) a[q][jn] = fma(-v[q], w, a[q][jn]);
+              reflector(jn, c1, readlane64(a[0][jn], c1), plain_norm2(jn, c1));
+              __builtin_amdgcn_s_setprio(0);
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          };
+          if (crit) {
+            if (o < NW - 1) critical(std::integral_constant<int, jc>{});
+            else critical(std::integral_constant<int, jc + 1>{});
+          }
+          // the rest: every local column's dot product in one transposed
+          // reduction; Gram entries of the finished columns, H_c applied to
+          // the unfinished ones (the critical column is done already)
+          double P[CPW];
+#pragma unroll
+          for (int jj = 0; jj < CPW; ++jj) {
+            double s0 = fma(v[0], a[0][jj], 0.0), s1 = 0.0;
+#pragma unroll
+            for (int q = 1; q < RPG; ++q) {
+              if (q & 1) s1 = fma(v[q], a[q][jj], s1);
+              else s0 = fma(v[q], a[q][jj], s0);
+            }
+            P[jj] = s0 + s1;
+          }
+          wave_reduce_stage<CPW, 0, CPW>(P, lane);
+          {
+            const int jj = wave_reduce_entry<CPW>(lane);
+            if ((lane & (64 / CPW - 1)) == 0 && wave + NW * jj < c) G[wave + NW * jj][c] = P[0];
+          }
+          const int skip = crit ? (o < NW - 1 ? jc : jc + 1) : -1;
+#pragma unroll
+          for (int jj = 0; jj < CPW; ++jj) {
+            if (jj != skip && wave + NW * jj > c && wave + NW * jj < nbj) {
+              const double w = tau * readlane64(P[0], wave_reduce_lane<CPW>(jj));
+#pragma unroll
+              for (int q = 0; q < RPG; ++q) a[q][jj] = fma(-v[q], w, a[q][jj]);
+            }
+          }
+        }
+      };
+      static_for<CPW>(column_round);
+    }
+    const bool any_stuck = __syncthreads_or(stuck) != 0;
+    if (any_stuck && tid == 0) *t.info = BSOFI_ERR_SCHEDULE;
+    return __syncthreads_or(bad && !any_stuck) != 0;
+  };
+  if (run(std::false_type{})) run(std::true_type{});
+
+  // G and taus are complete (the final barrier of run) and Vl is free for
+  // build_T's scratch.  The panel's stores go out first and stay in flight
+  // under build_T, whose barriers order LDS only.
+#pragma unroll
+  for (int q = 0; q < RPG; ++q) {
+    const int row = lane + 64 * q;
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int col = wave + NW * jj;
+      if (row < m && col < nbj) Pg[(size_t)col * lda + row] = a[q][jj];
+    }
+  }
+  if (tid < nbj) glb(t.tau)[tid] = taus[tid];
+  double (*Tl)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&Vl[0][0][0]);
+  build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Tl + NB);
+}
+
+template <int RPG, int CPW, int NW>
+__global__ __launch_bounds__(64 * NW) void panel_cyc_kernel(const PanelTask* __restrict__ tasks,
+                                                            const Item* __restrict__ items) {
+  panel_cyc_body<RPG, CPW, NW>(tasks[items[blockIdx.x].task]);
 }
 
 // ---------------------------------------------------------------------------
