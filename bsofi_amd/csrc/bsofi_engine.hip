@@ -1499,6 +1499,8 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
     P->d_blob = ctx_alloc(ctx, P->blob_bytes);
   }
   if (!P->d_blob) {
+    if (P->fork) (void)hipEventDestroy(P->fork);
+    if (P->join) (void)hipEventDestroy(P->join);
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
     return BSOFI_ERR_ALLOC;
@@ -1506,6 +1508,8 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
   if (!P->blob.h.empty() &&
       !hip_ok(hipMemcpyAsync(P->d_blob, P->blob.h.data(), P->blob.h.size(), hipMemcpyHostToDevice, ctx->stream),
               "hipMemcpyAsync(plan)")) {
+    if (P->fork) (void)hipEventDestroy(P->fork);
+    if (P->join) (void)hipEventDestroy(P->join);
     if (P->own_blob) ctx_free(ctx, P->d_blob, P->blob_bytes);
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
