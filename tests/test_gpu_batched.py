@@ -607,6 +607,47 @@ def test_device_schedule_same_bits(B, eps, case):
             assert oracle.residual(inputs[k], a) <= 100 * sizes[k][0] * sizes[k][1] * eps
 
 
+@pytest.mark.parametrize("scale", [1.0, 1e-300, 1e160])
+def test_cyclic_panel_same_bits_as_barrier_form(B, scale):
+    """The launch-per-step BSOF runs the cyclic sub-panel QR (no barrier per
+    column), the device-scheduled sweep the barrier form: the same FMA chains
+    and reduction tree, so the same A and tau bit for bit -- on the fast
+    dlarfg path (scale 1) and on LAPACK's scaled path (1e-300, 1e160, where
+    the whole panel runs again with the scaled norm).  Block sizes 16, 100
+    (its last sub-panels have 4 and 8 columns: waves without a column) and
+    256 (512-row panels), alone and in one plan."""
+    import torch
+
+    sp = [(4, 6), (10, 3), (16, 3)]
+    inputs = [dqmc.dqmc_matrix(lx, p, instance=90 + k)[0] * scale for k, (lx, p) in enumerate(sp)]
+    sizes = [(lx * lx, p) for lx, p in sp]
+
+    def bsof(schedule, idx):
+        ctx = B.Context(0)
+        ctx.set_schedule(schedule)
+        devs = [to_dev(inputs[k]) for k in idx]
+        taus = [torch.zeros(sizes[k][0] * sizes[k][1], dtype=torch.float64, device="cuda") for k in idx]
+        torch.cuda.synchronize()
+        plan = B.Plan(ctx, [(d.data_ptr(), t.data_ptr(), sizes[k][0], sizes[k][1], sizes[k][0] * sizes[k][1])
+                            for d, t, k in zip(devs, taus, idx)], phases=B.PHASE_BSOF)
+        plan.execute()
+        ctx.synchronize()
+        assert plan.info() == [0] * len(idx)
+        out = [(from_dev(d, sizes[k][0] * sizes[k][1]).copy(), t.cpu().numpy()) for d, t, k in zip(devs, taus, idx)]
+        plan.close()
+        ctx.close()
+        return out
+
+    print(f"cyclic vs barrier panel, scale {scale:g}: {sizes}", flush=True)
+    ref = bsof(1, [0, 1, 2])
+    for idx in ([0, 1, 2], [0], [1], [2]):
+        got = bsof(0, idx)
+        for (a, t), k in zip(got, idx):
+            assert np.all(np.isfinite(a)) and np.all(np.isfinite(t))
+            assert np.array_equal(a, ref[k][0]), f"{sizes[k]} scale {scale:g}: A differs"
+            assert np.array_equal(t, ref[k][1]), f"{sizes[k]} scale {scale:g}: tau differs"
+
+
 def test_device_schedule_wait_limit_reports_error(B):
     """A wait limit of zero makes the first unmet dependency stop the
     device-scheduled sweep: every workgroup leaves, every matrix of the plan
