@@ -322,10 +322,12 @@ class Context:
             lib.bsofi_ctx_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the HIP runtime may already be torn down:
         # leave the memory to the process exit instead of calling into it
-        if sys.is_finalizing():
+        # (sys.is_finalizing is bound when the class is defined: the module's
+        # globals, sys among them, may already be gone when this runs)
+        if _finalizing():
             return
         try:
             self.close()
@@ -396,10 +398,12 @@ class Plan:
             lib.bsofi_plan_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the HIP runtime may already be torn down:
         # leave the memory to the process exit instead of calling into it
-        if sys.is_finalizing():
+        # (sys.is_finalizing is bound when the class is defined: the module's
+        # globals, sys among them, may already be gone when this runs)
+        if _finalizing():
             return
         try:
             self.close()
