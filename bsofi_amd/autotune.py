@@ -133,8 +133,13 @@ def refine(B, ctx, table, descs, rebuild, reps=2, log=None, margin=0.005):
     plan is timed with up to three rows for that n: the tuned one, the
     library's defaults, and the fastest trial row with another scan form;
     the defaults stay unless another row is faster by more than `margin`.
-    Updates `table` in place (row["kept"]: "tuned", "default" or
-    "alternative") and the context's tuning."""
+    Then, on the row kept, 2 and 4 reflector slabs per workgroup (block
+    sizes up to n = 256, whose reflector steps take the multi-slab form):
+    a trial batch of one block size sees a different launch mix from the
+    whole plan's, where a count the trial rejected can pay.  A count is kept
+    only if faster by more than `margin` again.  Updates `table` in place
+    (row["kept"]: "tuned", "default" or "alternative", with "+spw2" / "+spw4"
+    where a slab count was kept) and the context's tuning."""
     for n in sorted(table, key=int, reverse=True):
         row = table[n]
         tuned = (int(row["nb"]), int(row["scan_mti"]), int(row.get("gemm_tile", 0)), int(row.get("refl_spw", 0)))
@@ -158,12 +163,25 @@ def refine(B, ctx, table, descs, rebuild, reps=2, log=None, margin=0.005):
         if times[best] >= (1.0 - margin) * times["default"]:
             best = "default"
         k = cands[best]
+        kept, best_t = best, times[best]
+        if int(n) <= 256:
+            for spw in (2, 4):
+                if k[3] == spw:
+                    continue
+                ks = (k[0], k[1], k[2], spw)
+                ctx.set_tuning(int(n), *ks)
+                name = f"{best}+spw{spw}"
+                cands[name] = ks
+                times[name] = time_plan(B, ctx, descs, reps, rebuild)
+                rebuild()
+                if times[name] < (1.0 - margin) * best_t:
+                    k, kept, best_t = ks, name, times[name]
         ctx.set_tuning(int(n), *k)
-        row.update(nb=k[0], scan_mti=k[1], gemm_tile=k[2], refl_spw=k[3], kept=best,
+        row.update(nb=k[0], scan_mti=k[1], gemm_tile=k[2], refl_spw=k[3], kept=kept,
                    whole_ms={name: round(t, 3) for name, t in times.items()})
         if log:
             log(f"[tune] whole workload, n={n}: " + ", ".join(f"{name} {cands[name]} {t:.2f} ms"
-                                                             for name, t in times.items()) + f" -> {best}")
+                                                             for name, t in times.items()) + f" -> {kept}")
     return table
 
 

@@ -185,9 +185,12 @@ def test_c4_uneven_tuned_batch(B, O, eps):
     # each tuned row against the defaults in the whole plan, then fresh inputs
     autotune.refine(B, ctx, table, mats.descs(), mats.rebuild, reps=1)
     mats.rebuild()
-    assert all(row["kept"] in ("tuned", "default", "alternative") for row in table.values())
-    print("  after the whole-plan pass: " + ", ".join(f"n={k}: scan_mti={v['scan_mti']} ({v['kept']})"
-                                                     for k, v in sorted(table.items())), flush=True)
+    for row in table.values():
+        base, _, spw = row["kept"].partition("+")
+        assert base in ("tuned", "default", "alternative") and spw in ("", "spw2", "spw4"), row["kept"]
+        assert not spw or int(row["refl_spw"]) == int(spw[3:])
+    print("  after the whole-plan pass: " + ", ".join(f"n={k}: scan_mti={v['scan_mti']} refl_spw={v['refl_spw']} "
+                                                     f"({v['kept']})" for k, v in sorted(table.items())), flush=True)
     # per lattice side: one matrix with p <= 32 (oracle) and the one with the largest p (alone only)
     oracle_k, alone_k = [], []
     for side in sorted({sp[0] for sp in specs}):
