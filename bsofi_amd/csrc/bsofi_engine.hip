@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "bsofi.h"
@@ -51,6 +52,11 @@ struct Launch {
   int spw = 1;   // reflector launches: slabs per workgroup
   bool prefork = false;  // runs on the context's stream before a two-lane plan forks
   size_t task_off = 0, item_off = 0, shmem = 0;
+  // panel and reflector launches of one task pass it in the kernel
+  // arguments (the kernels then read no task table)
+  bool one = false;
+  PanelTask pt1{};
+  ReflTask rt1{};
   // F_FLOW: blob offsets of the panel, reflector and matrix tables; the
   // zero-filled control region (queue head, stop word, counters, flags);
   // the plan's info words; the wait limit in wall-clock ticks
@@ -367,6 +373,20 @@ struct Planner {
     L.grid = (int)items.size();
     L.ldcar = ldcar;
     L.shmem = shmem;
+    // one task, item g = {0, g} (panels: g = 0; reflector slabs: emit_refl
+    // steps the slab by spw, workgroup g taking slab g spw)
+    if constexpr (std::is_same<T, PanelTask>::value) {
+      if (tasks.size() == 1 && items.size() == 1) {
+        L.one = true;
+        L.pt1 = tasks[0];
+      }
+    }
+    if constexpr (std::is_same<T, ReflTask>::value) {
+      if (tasks.size() == 1) {
+        L.one = true;
+        L.rt1 = tasks[0];
+      }
+    }
     push(L);
   }
 
@@ -851,7 +871,6 @@ struct Planner {
       emit_refl(rr);
     }
     // row scans, grouped by kernel instance; heaviest tiles first
-    struct Var { int mti, ntw; };
     std::vector<std::pair<int, int>> keys;
     for (auto& M : mats) keys.push_back({M.g.scan_mti, M.g.scan_ntw});
     std::sort(keys.begin(), keys.end());
@@ -1004,6 +1023,7 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     case F_PANEL: {
       auto t = (const PanelTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
+      const PanelTask& o = L.pt1;
       // variant = RPG * 100 + CPW
 #define PANEL_CASE(R, C, W)                                                                       \
   case W * 10000 + R * 100 + C:                                                                     \
@@ -1011,7 +1031,10 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     break;
 #define PANEL_CYC_CASE(R)                                                                                 \
   case kPanelCyc + 80000 + R * 100 + 4:                                                                   \
-    hipLaunchKernelGGL((panel_cyc_kernel<R, 4, 8>), dim3(L.grid), dim3(512), 0, s, t, it);                \
+    if (L.one)                                                                                            \
+      hipLaunchKernelGGL((panel_cyc_kernel<R, 4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, o);     \
+    else                                                                                                  \
+      hipLaunchKernelGGL((panel_cyc_kernel<R, 4, 8>), dim3(L.grid), dim3(512), 0, s, t, it, o);           \
     break;
       switch (L.variant) {
         PANEL_CYC_CASE(2) PANEL_CYC_CASE(3) PANEL_CYC_CASE(4) PANEL_CYC_CASE(5) PANEL_CYC_CASE(6)
@@ -1029,26 +1052,27 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
     case F_REFL: {
       auto t = (const ReflTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
+      const ReflTask& o = L.rt1;
+#define REFL_L(RT, MS)                                                                                       \
+  do {                                                                                                       \
+    if (L.one)                                                                                               \
+      hipLaunchKernelGGL((refl_apply_kernel<RT, 8, MS, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw, o); \
+    else                                                                                                     \
+      hipLaunchKernelGGL((refl_apply_kernel<RT, 8, MS>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw, o);   \
+  } while (0)
       switch (L.variant) {
-        case 101: hipLaunchKernelGGL((refl_apply_kernel<1, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
-        case 102: hipLaunchKernelGGL((refl_apply_kernel<2, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
-        case 103: hipLaunchKernelGGL((refl_apply_kernel<3, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
-        case 104: hipLaunchKernelGGL((refl_apply_kernel<4, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
-        case 201:
-          hipLaunchKernelGGL((refl_apply_kernel<1, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
-          break;
-        case 202:
-          hipLaunchKernelGGL((refl_apply_kernel<2, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
-          break;
-        case 203:
-          hipLaunchKernelGGL((refl_apply_kernel<3, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
-          break;
-        case 204:
-          hipLaunchKernelGGL((refl_apply_kernel<4, 8, true>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw);
-          break;
+        case 101: REFL_L(1, false); break;
+        case 102: REFL_L(2, false); break;
+        case 103: REFL_L(3, false); break;
+        case 104: REFL_L(4, false); break;
+        case 201: REFL_L(1, true); break;
+        case 202: REFL_L(2, true); break;
+        case 203: REFL_L(3, true); break;
+        case 204: REFL_L(4, true); break;
         case 116: hipLaunchKernelGGL(refl_apply_stream_kernel, dim3(L.grid), dim3(512), 0, s, t, it); break;
-        default: hipLaunchKernelGGL((refl_apply_kernel<8, 8>), dim3(L.grid), dim3(512), 0, s, t, it, L.spw); break;
+        default: REFL_L(8, false); break;
       }
+#undef REFL_L
       break;
     }
     case F_GEMM:
@@ -1331,6 +1355,16 @@ size_t bsofi_plan_workspace_bytes(int count, const bsofi_matrix_t* mats, int pha
 // Plan creation.  blob_dev, when given, is device memory of blob_cap bytes
 // for the task tables (else they come from the context's buffer cache); a
 // plan whose tables do not fit returns BSOFI_ERR_SCRATCH.  one_lane keeps
+// The plan's fork/join events (two-lane plans).
+static void plan_free_events(bsofi_plan_s* P) {
+  hipEvent_t* ev[] = {&P->fork, &P->join};
+  for (hipEvent_t* e : ev)
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
+}
+
 // every launch on the context's stream (no side stream or events).
 static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, int phases, void* workspace,
                        char* blob_dev, size_t blob_cap, bool one_lane, bsofi_plan_t* out) {
@@ -1463,8 +1497,7 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
     ok = ok && hip_ok(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming), "hipEventCreate");
     ok = ok && hip_ok(hipEventCreateWithFlags(&P->join, hipEventDisableTiming), "hipEventCreate");
     if (!ok) {
-      if (P->fork) (void)hipEventDestroy(P->fork);
-      if (P->join) (void)hipEventDestroy(P->join);
+      plan_free_events(P);
       if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
       delete P;
       return BSOFI_ERR_NO_DEVICE;
@@ -1473,8 +1506,7 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
   for (auto& L : P->launches) {
     if (!known_variant(L)) {
       fprintf(stderr, "bsofi_amd: no kernel instance for family %d variant %d\n", L.family, L.variant);
-      if (P->fork) (void)hipEventDestroy(P->fork);
-      if (P->join) (void)hipEventDestroy(P->join);
+      plan_free_events(P);
       if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
       delete P;
       return BSOFI_ERR_UNSUPPORTED;
@@ -1487,8 +1519,7 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
   P->blob_bytes = std::max<size_t>(P->blob.h.size(), 256);
   if (blob_dev) {
     if (P->blob_bytes > blob_cap) {
-      if (P->fork) (void)hipEventDestroy(P->fork);
-      if (P->join) (void)hipEventDestroy(P->join);
+      plan_free_events(P);
       if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
       delete P;
       return BSOFI_ERR_SCRATCH;
@@ -1499,8 +1530,7 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
     P->d_blob = ctx_alloc(ctx, P->blob_bytes);
   }
   if (!P->d_blob) {
-    if (P->fork) (void)hipEventDestroy(P->fork);
-    if (P->join) (void)hipEventDestroy(P->join);
+    plan_free_events(P);
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
     return BSOFI_ERR_ALLOC;
@@ -1508,8 +1538,7 @@ static int plan_create(bsofi_ctx_t ctx, int count, const bsofi_matrix_t* mats, i
   if (!P->blob.h.empty() &&
       !hip_ok(hipMemcpyAsync(P->d_blob, P->blob.h.data(), P->blob.h.size(), hipMemcpyHostToDevice, ctx->stream),
               "hipMemcpyAsync(plan)")) {
-    if (P->fork) (void)hipEventDestroy(P->fork);
-    if (P->join) (void)hipEventDestroy(P->join);
+    plan_free_events(P);
     if (P->own_blob) ctx_free(ctx, P->d_blob, P->blob_bytes);
     if (P->own_ws) ctx_free(ctx, P->ws, P->ws_block);
     delete P;
@@ -1691,8 +1720,7 @@ int bsofi_plan_destroy(bsofi_plan_t P) {
   (void)hipStreamSynchronize(P->ctx->stream);
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
-  if (P->fork) (void)hipEventDestroy(P->fork);
-  if (P->join) (void)hipEventDestroy(P->join);
+  plan_free_events(P);
   for (auto e : P->ev) (void)hipEventDestroy(e);
   // the stream has drained: the buffers go back to the context for reuse
   if (P->own_blob) ctx_free(P->ctx, P->d_blob, P->blob_bytes);

@@ -505,6 +505,18 @@ __global__ __launch_bounds__(64 * NW) void panel_factor_kernel(const PanelTask* 
   panel_body<RPG, CPW, NW>(tasks[items[blockIdx.x].task]);
 }
 
+// A launch's task: from the task table through the workgroup's item, or,
+// in the ONE instances (launches of one task), from the kernel arguments.
+// The single-matrix path launches one task at a time: its first data load
+// then waits for the kernel argument load alone, not for the item and task
+// reads behind it.
+template <bool ONE, class T>
+__device__ __forceinline__ const T& launch_task(const T* __restrict__ tasks, const Item* __restrict__ items,
+                                                const T& one) {
+  if constexpr (ONE) return one;
+  else return tasks[items[blockIdx.x].task];
+}
+
 // f(std::integral_constant<int, i>{}) for i = 0 .. N-1 in order.
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
@@ -746,10 +758,11 @@ __device__ __forceinline__ void panel_cyc_body(const PanelTask& t) {
   build_T<NB, NT>(G, taus, nbj, glb(t.T), t.ldt, Tl, Tl + NB);
 }
 
-template <int RPG, int CPW, int NW>
+template <int RPG, int CPW, int NW, bool ONE = false>
 __global__ __launch_bounds__(64 * NW) void panel_cyc_kernel(const PanelTask* __restrict__ tasks,
-                                                            const Item* __restrict__ items) {
-  panel_cyc_body<RPG, CPW, NW>(tasks[items[blockIdx.x].task]);
+                                                            const Item* __restrict__ items,
+                                                            const PanelTask one) {
+  panel_cyc_body<RPG, CPW, NW>(launch_task<ONE>(tasks, items, one));
 }
 
 // ---------------------------------------------------------------------------
@@ -967,11 +980,13 @@ __device__ __forceinline__ void refl_body(const ReflTask& t, const Item it, int 
   }
 }
 
-template <int RT, int NW, bool MS = false>
+template <int RT, int NW, bool MS = false, bool ONE = false>
 __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __restrict__ tasks,
-                                                             const Item* __restrict__ items, int spw) {
-  const Item it = items[blockIdx.x];
-  refl_body<RT, NW, MS>(tasks[it.task], it, spw);
+                                                             const Item* __restrict__ items, int spw,
+                                                             const ReflTask one) {
+  // one task: workgroup g takes slabs g spw .. g spw + spw - 1
+  const Item it = ONE ? Item{0, (int)blockIdx.x * spw} : items[blockIdx.x];
+  refl_body<RT, NW, MS>(launch_task<ONE>(tasks, items, one), it, spw);
 }
 
 // ---------------------------------------------------------------------------
