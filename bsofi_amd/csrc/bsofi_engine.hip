@@ -373,18 +373,12 @@ struct Planner {
     L.grid = (int)items.size();
     L.ldcar = ldcar;
     L.shmem = shmem;
-    // one task, item g = {0, g} (panels: g = 0; reflector slabs: emit_refl
-    // steps the slab by spw, workgroup g taking slab g spw)
+    // a panel launch of one task: its one workgroup takes the task from the
+    // kernel arguments (reflector launches: emit_refl, which knows spw)
     if constexpr (std::is_same<T, PanelTask>::value) {
-      if (tasks.size() == 1 && items.size() == 1) {
+      if (tasks.size() == 1 && items.size() == 1 && items[0].task == 0) {
         L.one = true;
         L.pt1 = tasks[0];
-      }
-    }
-    if constexpr (std::is_same<T, ReflTask>::value) {
-      if (tasks.size() == 1) {
-        L.one = true;
-        L.rt1 = tasks[0];
       }
     }
     push(L);
@@ -453,7 +447,19 @@ struct Planner {
     }
     const size_t before = out.size();
     emit(F_REFL, variant, R.t, items);
-    if (out.size() > before) out.back().spw = std::max(spw, 1);
+    if (out.size() > before) {
+      Launch& L = out.back();
+      L.spw = std::max(spw, 1);
+      // one task whose workgroup g takes slab g spw: the task goes in the
+      // kernel arguments, and the kernel forms the item itself
+      bool one = R.t.size() == 1 && variant != 116;
+      for (size_t g = 0; one && g < items.size(); ++g)
+        one = items[g].task == 0 && items[g].sub == (int)g * L.spw;
+      if (one) {
+        L.one = true;
+        L.rt1 = R.t[0];
+      }
+    }
   }
 
   // -- BSOF: src/bsofi_cpu.c:86-118 ---------------------------------------

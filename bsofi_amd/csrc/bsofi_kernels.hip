@@ -833,9 +833,14 @@ __device__ __forceinline__ void refl_body(const ReflTask& t, const Item it, int 
   gptr<double> cp[4];
   columns(it.sub, cp);
   const gptr<const double> Vg = glb(t.V);
-  for (int e = tid; e < 32 * 32; e += NT) {
-    const int i = e & 31, j = e >> 5;
-    Tl[i][j] = (i < nbj && j < nbj) ? glb(t.T)[(size_t)j * t.ldt + i] : 0.0;
+  // T goes to registers first and into LDS once the C and V loads are out,
+  // so that their latencies overlap (the vector memory counter is in order)
+  constexpr int TE = 32 * 32 / NT;
+  double tv[TE];
+#pragma unroll
+  for (int q = 0; q < TE; ++q) {
+    const int e = tid + q * NT, i = e & 31, j = e >> 5;
+    tv[q] = (i < nbj && j < nbj) ? glb(t.T)[(size_t)j * t.ldt + i] : 0.0;
   }
   // Registers hold V up front for up to 4 row tiles per wave; beyond that
   // (8 waves x 8 tiles, m <= 1024) V tiles are loaded when used (same
@@ -859,6 +864,11 @@ __device__ __forceinline__ void refl_body(const ReflTask& t, const Item it, int 
 #pragma unroll
       for (int kq = 0; kq < 8; ++kq) vb[s][kq] = vload(Vg, t.ldv, m, nbj, row, 4 * kq + lk);
     }
+  }
+#pragma unroll
+  for (int q = 0; q < TE; ++q) {
+    const int e = tid + q * NT;
+    Tl[e & 31][e >> 5] = tv[q];
   }
   auto vtile = [&](int s, int kq) -> double {
     if constexpr (PREV) return vb[s][kq];
