@@ -277,7 +277,7 @@ def bench_uneven(B, count, seed, reps, tune_p, log_fn, save_tune=None):
     mats = workload.DeviceMatrices(B, ctx, specs, seed=seed)
     # each tuned block size against the defaults, in the whole plan
     t0 = time.perf_counter()
-    autotune.refine(B, ctx, table, mats.descs(), mats.rebuild, reps=2, log=log_fn)
+    autotune.refine(B, ctx, table, mats.descs(), mats.rebuild, reps=3, log=log_fn)
     tune_s += time.perf_counter() - t0
     mats.rebuild()
     if save_tune:
