@@ -838,6 +838,10 @@ struct Planner {
       et.push_back({M.Q, (long long)4 * g.n * g.n, 2 * g.n, g.p - 1});
     }
     emit(F_LARFT, 0, lt, li);
+    if (emit_qform(mats)) {
+      bsoi_scans(mats, max_shmem, diag_only);
+      return;
+    }
     if (!et.empty()) {
       Launch L;
       L.family = F_EYE;
@@ -876,7 +880,59 @@ struct Planner {
       }
       emit_refl(rr);
     }
-    // row scans, grouped by kernel instance; heaviest tiles first
+    bsoi_scans(mats, max_shmem, diag_only);
+  }
+
+  // Every Q_k of the plan in one launch (qform_kernel), where every matrix
+  // has 2n <= 512 and 16 | nb: the identity fill and the S reflector rounds
+  // of the launch list, which write and re-read Q_k once per round, with
+  // the same bits.  Slabs heaviest first (a slab takes every H_j with
+  // j nb <= its first column), and all slabs of one Q_k on one XCD, which
+  // reads the V_j through one L2.
+  bool emit_qform(const std::vector<MatJob>& mats) {
+    int rt = 1;
+    for (auto& M : mats) {
+      const Geo& g = M.g;
+      if (2 * g.n > 512 || (g.nb != 16 && g.nb != 32)) return false;
+      rt = std::max(rt, (2 * g.n + 127) / 128);
+    }
+    std::vector<QformTask> qt;
+    struct W { long long work; Item it; };
+    std::vector<W> order;
+    for (auto& M : mats) {
+      const Geo& g = M.g;
+      const int n2 = 2 * g.n;
+      for (int k = 0; k <= g.p - 2; ++k) {
+        const int ncols = (k == g.p - 2) ? n2 : g.n;
+        const int S = (ncols + g.nb - 1) / g.nb;
+        const int off = k * g.n;
+        const int id = (int)qt.size();
+        qt.push_back({M.A + (size_t)off * g.lda + off, Tptr(M, k, 0), M.Q + (size_t)k * 4 * g.n * g.n, g.lda, n2,
+                      g.nb, S, ncols});
+        for (int slab = 0; slab * 16 < n2; ++slab) {
+          long long work = 0;
+          for (int j = std::min(S - 1, (16 * slab) / g.nb); j >= 0; --j) work += n2 - j * g.nb;
+          order.push_back({work, {id, slab}});
+        }
+      }
+    }
+    if (qt.empty()) return true;
+    std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.work > b.work; });
+    constexpr int kXcd = 8;
+    std::vector<std::vector<Item>> q(kXcd);
+    for (const W& w : order) q[w.it.task % kXcd].push_back(w.it);
+    std::vector<Item> items;
+    items.reserve(order.size());
+    std::vector<size_t> head(kXcd, 0);
+    while (items.size() < order.size())
+      for (int x = 0; x < kXcd; ++x)
+        if (head[x] < q[x].size()) items.push_back(q[x][head[x]++]);
+    emit(F_REFL, 400 + rt, qt, items);
+    return true;
+  }
+
+  // (4) row scans, grouped by kernel instance; heaviest tiles first
+  void bsoi_scans(std::vector<MatJob>& mats, int max_shmem, bool diag_only) {
     std::vector<std::pair<int, int>> keys;
     for (auto& M : mats) keys.push_back({M.g.scan_mti, M.g.scan_ntw});
     std::sort(keys.begin(), keys.end());
@@ -1056,8 +1112,18 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_REFL: {
-      auto t = (const ReflTask*)(base + L.task_off);
       auto it = (const Item*)(base + L.item_off);
+      if (L.variant > 400 && L.variant <= 404) {
+        auto qt = (const QformTask*)(base + L.task_off);
+        switch (L.variant) {
+          case 401: hipLaunchKernelGGL(qform_kernel<1>, dim3(L.grid), dim3(512), 0, s, qt, it); break;
+          case 402: hipLaunchKernelGGL(qform_kernel<2>, dim3(L.grid), dim3(512), 0, s, qt, it); break;
+          case 403: hipLaunchKernelGGL(qform_kernel<3>, dim3(L.grid), dim3(512), 0, s, qt, it); break;
+          default: hipLaunchKernelGGL(qform_kernel<4>, dim3(L.grid), dim3(512), 0, s, qt, it); break;
+        }
+        break;
+      }
+      auto t = (const ReflTask*)(base + L.task_off);
       const ReflTask& o = L.rt1;
 #define REFL_L(RT, MS)                                                                                       \
   do {                                                                                                       \
@@ -1150,7 +1216,7 @@ static bool known_variant(const Launch& L) {
     }
     case F_REFL:
       return (L.variant >= 101 && L.variant <= 104) || (L.variant >= 201 && L.variant <= 204) || L.variant == 108 ||
-             L.variant == 116;
+             L.variant == 116 || (L.variant >= 401 && L.variant <= 404);
     case F_GEMM:
       return L.variant >= 0 && L.variant <= 2;
     case F_FLOW:

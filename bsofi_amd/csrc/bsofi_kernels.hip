@@ -1000,6 +1000,155 @@ __global__ __launch_bounds__(64 * NW) void refl_apply_kernel(const ReflTask* __r
 }
 
 // ---------------------------------------------------------------------------
+// Explicit Q_k in one launch (BSOI's dorgqr)
+// ---------------------------------------------------------------------------
+// Q_k = H_0 ... H_{S-1} applied to the identity backward (j = S-1 .. 0),
+// H_j = I - V_j T_j V_j^T acting on rows and columns >= j nb.  One
+// workgroup (8 waves) per 16-column slab of Q_k keeps the slab's n2 rows
+// in MFMA accumulators (RT row tiles per wave, wave w holding tiles w + 8 s)
+// from the identity to the end, and applies every H_j that reaches the slab
+// (j nb <= its first column) with refl_body's arithmetic: the reflector
+// rounds of one launch each, from and to memory, give the same bits.  Round
+// j counts row tiles from its first row (t0 = j nb / 16), so relative tile
+// t = t_abs - t0 belongs to chain ((t mod 4), (t / 4) mod 2); every tile a
+// wave holds has the same t mod 8, so each wave forms one chain, in
+// increasing t, and phase 2 adds the chains in refl_body's order.
+template <int RT>
+__global__ __launch_bounds__(512) void qform_kernel(const QformTask* __restrict__ tasks,
+                                                    const Item* __restrict__ items) {
+  constexpr int NW = 8, NT = 512;
+  const Item it = items[blockIdx.x];
+  const QformTask t = tasks[it.task];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int n2 = t.n2, nb = t.nb, slab = it.sub;
+
+  __shared__ double Wf[32][17];
+  __shared__ double Tl[32][33];
+  __shared__ double XW[NW][32 * 17];
+  __shared__ double W[32][17];
+  auto Wp = [&](int w) { return reinterpret_cast<double (*)[17]>(XW[w]); };
+  double (*X)[33] = reinterpret_cast<double (*)[33]>(XW[wave]);
+
+  d4 acc[RT];  // C[4r + lk][li] of row tile wave + 8 s: the identity to start
+#pragma unroll
+  for (int s = 0; s < RT; ++s)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = (wave + NW * s) * 16 + 4 * r + lk, col = 16 * slab + li;
+      acc[s][r] = (row == col && col < n2) ? 1.0 : 0.0;
+    }
+
+  const int jmax = min(t.S - 1, (16 * slab) / nb);
+  for (int j = jmax; j >= 0; --j) {
+    const int m = n2 - j * nb, nbj = min(nb, t.ncols - j * nb), t0 = (j * nb) / 16;
+    const gptr<const double> Vg = glb(t.V0) + (size_t)j * nb * (t.ldv + 1);
+    const gptr<const double> Tg = glb(t.T0) + (size_t)j * nb * nb;
+    const bool two = nbj > 16;
+    // T to registers, V tiles of the rows this reflector reaches, then T to LDS
+    constexpr int TE = 32 * 32 / NT;
+    double tv[TE];
+#pragma unroll
+    for (int q = 0; q < TE; ++q) {
+      const int e = tid + q * NT, i = e & 31, c = e >> 5;
+      tv[q] = (i < nbj && c < nbj) ? Tg[(size_t)c * nb + i] : 0.0;
+    }
+    double vb[RT][8];  // V[t * 16 + li][4 kq + lk], t = wave + 8 s - t0 (relative tile)
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      const int trel = wave + NW * s - t0;
+#pragma unroll
+      for (int kq = 0; kq < 8; ++kq)
+        vb[s][kq] = trel >= 0 ? vload(Vg, t.ldv, m, nbj, trel * 16 + li, 4 * kq + lk) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < TE; ++q) {
+      const int e = tid + q * NT;
+      Tl[e & 31][e >> 5] = tv[q];
+    }
+    // phase 1: W = V^T C, this wave's one chain
+    d4 w0a = d4zero(), w1a = d4zero();
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      const int trel = wave + NW * s - t0;
+      if (trel < 0) continue;
+#pragma unroll
+      for (int kq = 0; kq < 8; ++kq) X[li][4 * kq + lk] = vb[s][kq];
+      double va[4][2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        va[r][0] = X[4 * r + lk][li];
+        va[r][1] = X[4 * r + lk][16 + li];
+      }
+      if (trel * 16 < m) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          w0a = mfma(va[r][0], acc[s][r], w0a);
+          if (two) w1a = mfma(va[r][1], acc[s][r], w1a);
+        }
+      }
+    }
+    {
+      double (*Wq)[17] = Wp(wave);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Wq[crow(lk, r)][li] = w0a[r];
+        Wq[16 + crow(lk, r)][li] = two ? w1a[r] : 0.0;
+      }
+    }
+    __syncthreads();
+    // phase 2: W = sum over q of (chain (q, 0) + chain (q, 1)); chain c is
+    // in wave (c + t0) mod 8
+    for (int e = tid; e < 32 * 16; e += NT) {
+      const int i = e >> 4, c = e & 15;
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v += Wp((q + t0) & 7)[i][c] + Wp((q + 4 + t0) & 7)[i][c];
+      W[i][c] = v;
+    }
+    __syncthreads();
+    // Wf = -T W (waves 0 and 1: rows 0-15 and 16-31, K = 32)
+    if (wave < 2) {
+      const int i = wave * 16 + li;
+      d4 tw = d4zero();
+#pragma unroll
+      for (int kc = 0; kc < 8; ++kc) {
+        const int k = 4 * kc + lk;
+        tw = mfma(Tl[i][k], W[k][li], tw);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Wf[wave * 16 + crow(lk, r)][li] = -tw[r];
+    }
+    __syncthreads();
+    // phase 3: C += V Wf on the rows this reflector reaches
+#pragma unroll
+    for (int s = 0; s < RT; ++s) {
+      const int trel = wave + NW * s - t0;
+      if (trel >= 0 && trel * 16 < m) {
+#pragma unroll
+        for (int kq = 0; kq < 8; ++kq)
+          if (4 * kq < nbj) acc[s] = mfma(vb[s][kq], Wf[4 * kq + lk][li], acc[s]);
+      }
+    }
+    __syncthreads();  // LDS (Tl, W, Wf, the wave tiles) free for the next reflector
+  }
+  // the slab, back to the row-major-lanes layout and out
+#pragma unroll
+  for (int s = 0; s < RT; ++s) {
+    const int rt = wave + NW * s;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[crow(lk, r)][li] = acc[s][r];
+    const int row = rt * 16 + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double v = X[li][4 * r + lk];
+      const int col = 16 * slab + 4 * r + lk;
+      if (row < n2 && col < n2) glb(t.Q)[(size_t)col * n2 + row] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Device-scheduled BSOF sweep
 // ---------------------------------------------------------------------------
 // Number of sub-panels before g (step-major index within one matrix) whose

@@ -35,6 +35,19 @@ struct ReflTask {
   int m, nbj, w1, w2, trans;
 };
 
+// Explicit Q_k = H_0 H_1 ... H_{S-1} (dorgqr, backward accumulation) of one
+// BSOF step from its sub-panels' block reflectors: sub-panel j has
+// V_j = V0 + j nb (ldv + 1) (m = n2 - j nb rows, nbj = min(nb, ncols - j nb)
+// columns) and T_j = T0 + j nb nb (leading dimension nb); Q is n2 x n2 with
+// leading dimension n2.  A workgroup builds one 16-column slab of Q from
+// the identity, every H_j that reaches it applied in registers.
+struct QformTask {
+  const double* V0;
+  const double* T0;
+  double* Q;
+  int ldv, n2, nb, S, ncols;
+};
+
 // Batched tile GEMM: C = alpha*A*B + beta*C (+ D on columns >= dcol0).
 // Triangular operands are read through masks: amask = 1 treats A as upper
 // triangular (A[i][k] = 0 for i > k); bmask = 1 treats B as upper
