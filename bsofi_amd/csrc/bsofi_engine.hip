@@ -347,6 +347,24 @@ static double* Tptr(const MatJob& m, int k, int j) {
   return m.T + ((size_t)k * m.g.S + j) * m.g.nb * m.g.nb;
 }
 
+// Workgroup i runs on XCD i mod 8 (each XCD with its own L2): reorders
+// `items` of a launch of several tasks so that every item of task t runs on
+// XCD t mod 8, in their order within each XCD (a task's workgroups then read
+// its shared operands through one L2).
+static void deal_tasks_to_xcds(std::vector<Item>& items, size_t ntasks) {
+  constexpr int kXcd = 8;
+  if (ntasks < 2) return;
+  std::vector<std::vector<Item>> q(kXcd);
+  for (const Item& it : items) q[it.task % kXcd].push_back(it);
+  std::vector<Item> dealt;
+  dealt.reserve(items.size());
+  std::vector<size_t> head(kXcd, 0);
+  while (dealt.size() < items.size())
+    for (int x = 0; x < kXcd; ++x)
+      if (head[x] < q[x].size()) dealt.push_back(q[x][head[x]++]);
+  items.swap(dealt);
+}
+
 struct Planner {
   Blob& blob;
   std::vector<Launch>& out;
@@ -429,22 +447,10 @@ struct Planner {
     std::vector<Item> items;
     for (size_t k = 0; k < R.t.size(); ++k)
       for (int s = 0; s < R.slabs[k]; s += spw) items.push_back({(int)k, s});
-    // Workgroup i runs on XCD i mod 8: deal the tasks to XCDs round-robin and
-    // interleave, so that all workgroups of a task run on one XCD and read
-    // its V through one L2 (task by task, a task's slabs spread over every
-    // XCD and each fetched V from HBM).
-    if (R.t.size() > 1) {
-      constexpr int kXcd = 8;
-      std::vector<std::vector<Item>> q(kXcd);
-      for (const Item& it : items) q[it.task % kXcd].push_back(it);
-      std::vector<Item> dealt;
-      dealt.reserve(items.size());
-      std::vector<size_t> head(kXcd, 0);
-      while (dealt.size() < items.size())
-        for (int x = 0; x < kXcd; ++x)
-          if (head[x] < q[x].size()) dealt.push_back(q[x][head[x]++]);
-      items.swap(dealt);
-    }
+    // all workgroups of a task on one XCD, reading its V through one L2
+    // (task by task, a task's slabs spread over every XCD and each fetched V
+    // from HBM)
+    deal_tasks_to_xcds(items, R.t.size());
     const size_t before = out.size();
     emit(F_REFL, variant, R.t, items);
     if (out.size() > before) {
@@ -918,15 +924,10 @@ struct Planner {
     }
     if (qt.empty()) return true;
     std::stable_sort(order.begin(), order.end(), [](const W& a, const W& b) { return a.work > b.work; });
-    constexpr int kXcd = 8;
-    std::vector<std::vector<Item>> q(kXcd);
-    for (const W& w : order) q[w.it.task % kXcd].push_back(w.it);
     std::vector<Item> items;
     items.reserve(order.size());
-    std::vector<size_t> head(kXcd, 0);
-    while (items.size() < order.size())
-      for (int x = 0; x < kXcd; ++x)
-        if (head[x] < q[x].size()) items.push_back(q[x][head[x]++]);
+    for (const W& w : order) items.push_back(w.it);
+    deal_tasks_to_xcds(items, qt.size());
     emit(F_REFL, 400 + rt, qt, items);
     return true;
   }
