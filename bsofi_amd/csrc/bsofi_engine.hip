@@ -84,7 +84,7 @@ struct Geo {
   int n, p, lda, N, nb, S;
   int panel_variant;  // panel_factor_kernel instance
   int scan_mti, scan_ntw;
-  int gemm_tile;  // BSTRI GEMM tile edge: 0 = the launch-size rule, else 32, 64 or 128
+  int gemm_tile;  // BSTRI GEMM tile edge: 0 = the launch-size rule (row sweep by strips where it applies), else 32, 64 or 128
   int refl_spw;   // reflector slabs per workgroup: 0 = the launch-size rule, else 1, 2, 4 or 8
 };
 
@@ -732,6 +732,38 @@ struct Planner {
     return t;
   }
 
+  // The row sweep of every matrix with 128 <= n <= 256, 32 | n, as one
+  // launch of column-strip chains (sweep_kernel): each strip of block column
+  // b runs its b products in turn, the strips side by side, longest chain
+  // first, and the strips of one block column on one XCD (they read the same
+  // W_i at the same time).  The strips' elements are the GEMM rounds' to the
+  // bit, so which matrices take it never changes a result.  Below n = 128 the
+  // rounds stay: their launches hold many small products at full occupancy,
+  // while a strip's steps are short and latency-bound (the 512-matrix n = 64
+  // batch ran 3.6 % slower through strips).  A matrix whose tuning forces
+  // a GEMM tile keeps the rounds with that tile.  Returns the matrices left.
+  std::vector<MatJob*> emit_sweep(std::vector<MatJob>& mats) {
+    std::vector<MatJob*> rest;
+    std::vector<SweepTask> st;
+    int rtw = 1;
+    for (auto& M : mats) {
+      const Geo& g = M.g;
+      if (g.n % 32 != 0 || g.n < 128 || g.n > 256 || g.gemm_tile != 0) {
+        rest.push_back(&M);
+        continue;
+      }
+      rtw = std::max(rtw, (g.n + 127) / 128);
+      for (int b = 1; b < g.p; ++b) st.push_back({M.A, M.W, M.Z2, g.lda, g.n, g.p, b});
+    }
+    std::stable_sort(st.begin(), st.end(), [](const SweepTask& x, const SweepTask& y) { return x.b > y.b; });
+    std::vector<Item> items;
+    for (size_t id = 0; id < st.size(); ++id)
+      for (int c = 0; c < st[id].n / 32; ++c) items.push_back({(int)id, c});
+    deal_tasks_to_xcds(items, st.size());
+    emit(F_GEMM, 10 + rtw, st, items);
+    return rest;
+  }
+
   void bstri(std::vector<MatJob>& mats) {
     // (1) X_ii = R_ii^{-1} for every diagonal block: 32 x 32 diagonal
     // sub-blocks first, then block columns J = 1.. by two GEMMs each:
@@ -799,11 +831,13 @@ struct Planner {
     }
     // (3) Row sweep, bottom-up: X_{i,i+1:p-1} = W_i X_{i+1,i+1:p-1}, plus
     // Z2_i on the last block column (src/bsofi_cpu.c:161-167).
+    const std::vector<MatJob*> rest = emit_sweep(mats);
     int rounds = 0;
-    for (auto& M : mats) rounds = std::max(rounds, M.g.p - 1);
+    for (MatJob* M : rest) rounds = std::max(rounds, M->g.p - 1);
     for (int s = 0; s < rounds; ++s) {
       GemmBatch b;
-      for (auto& M : mats) {
+      for (MatJob* Mp : rest) {
+        const MatJob& M = *Mp;
         const Geo& g = M.g;
         const int i = g.p - 2 - s;
         if (i < 0) continue;
@@ -1149,7 +1183,14 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_GEMM:
-      if (L.variant == 2)
+      if (L.variant == 11 || L.variant == 12) {
+        auto st = (const SweepTask*)(base + L.task_off);
+        auto si = (const Item*)(base + L.item_off);
+        if (L.variant == 11)
+          hipLaunchKernelGGL(sweep_kernel<1>, dim3(L.grid), dim3(512), 0, s, st, si);
+        else
+          hipLaunchKernelGGL(sweep_kernel<2>, dim3(L.grid), dim3(512), 0, s, st, si);
+      } else if (L.variant == 2)
         hipLaunchKernelGGL(gemm_kernel<1>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
                            (const Item*)(base + L.item_off));
       else if (L.variant == 1)
@@ -1219,7 +1260,7 @@ static bool known_variant(const Launch& L) {
       return (L.variant >= 101 && L.variant <= 104) || (L.variant >= 201 && L.variant <= 204) || L.variant == 108 ||
              L.variant == 116 || (L.variant >= 401 && L.variant <= 404);
     case F_GEMM:
-      return L.variant >= 0 && L.variant <= 2;
+      return (L.variant >= 0 && L.variant <= 2) || L.variant == 11 || L.variant == 12;
     case F_FLOW:
       return L.variant == 21 || L.variant == 42 || L.variant == 84;
     case F_SCAN:

@@ -1594,6 +1594,113 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// BSTRI row sweep by column strips (K4s)
+// ---------------------------------------------------------------------------
+// One workgroup (8 waves) per 32-column strip of a block column b >= 1 runs
+// the strip's whole chain X_{i,b} = W_i X_{i+1,b} (+ Z2_i) for
+// i = b - 1 .. 0: the strip of X_{i+1,b} (n x 32) is the product's B, in
+// LDS, and W_i streams from L2 into the A fragments, so the sweep's p - 1
+// dependent GEMM launches become one launch whose strips run side by side.
+// Every element is gemm_kernel's: one v_mfma_f64_16x16x4f64 chain over k in
+// steps of 4, increasing, times alpha = 1, plus Z2_i on the last block
+// column, the diagonal block read through the same upper-triangular mask;
+// so the bits are the same.  The planner deals the strips of one block
+// column to one XCD: they start together and step through the same W_i,
+// which each XCD's L2 then fetches once.  Wave w holds row tiles w, w + 8 (RTW of them:
+// n <= 128 RTW, 32 | n) and both 16-column tiles of the strip.  The strip
+// is double-buffered: step i reads one buffer and writes its result to the
+// other, one barrier a step; the 2 x 65 KiB also keep one workgroup per CU,
+// so that a long chain never shares its CU's matrix cores.
+template <int RTW>
+__global__ __launch_bounds__(512) void sweep_kernel(const SweepTask* __restrict__ tasks,
+                                                    const Item* __restrict__ items) {
+  constexpr int NW = 8, SLD = 260, PD = 8;  // PD: k-steps of W_i in flight
+  const Item it = items[blockIdx.x];
+  const SweepTask t = tasks[it.task];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const int n = t.n, p = t.p, lda = t.lda;
+  const int b = t.b, c0 = it.sub * 32;
+  __shared__ double Bs[2][32][SLD];  // B[k][col] at Bs[buf][col][k]
+  const gptr<double> A = glb(t.A);
+  for (int e = tid; e < 32 * n; e += NW * 64) {
+    const int k = e % n, col = e / n;
+    Bs[0][col][k] = (k <= c0 + col) ? A[(size_t)(b * n + c0 + col) * lda + b * n + k] : 0.0;
+  }
+  // Row tiles of this wave (uniform), and each lane's offset into W_i: a
+  // tile beyond n reads tile 0's rows (in bounds) and is never multiplied.
+  bool has[RTW];
+  int woff[RTW];
+#pragma unroll
+  for (int s = 0; s < RTW; ++s) {
+    const int rt = wave + NW * s;
+    has[s] = rt * 16 < n;
+    woff[s] = lk * n + (has[s] ? rt * 16 : 0) + li;
+  }
+  // A fragments W_i[rt 16 + li][k + lk], PD k-steps ahead across steps
+  const gptr<const double> W = glb(t.W);
+  const size_t nn = (size_t)n * n;
+  double a[PD][RTW];
+  auto load = [&](int u, const gptr<const double> Wk) {
+#pragma unroll
+    for (int s = 0; s < RTW; ++s) a[u][s] = Wk[woff[s]];
+  };
+#pragma unroll
+  for (int u = 0; u < PD; ++u) load(u, W + (size_t)(b - 1) * nn + 4 * u * n);
+  __syncthreads();
+  int cur = 0;
+  for (int i = b - 1; i >= 0; --i) {
+    const gptr<const double> Wi = W + (size_t)i * nn;
+    d4 acc[RTW][2];
+#pragma unroll
+    for (int s = 0; s < RTW; ++s) acc[s][0] = acc[s][1] = d4zero();
+    for (int k0 = 0; k0 < n; k0 += 4 * PD) {
+      // the chunk the slots load next: the rest of W_i, then the start of
+      // W_{i-1} (4 PD | n); after W_0 they reload W_0's first rows (in
+      // bounds, never used): the loads stay unconditional, so the compiler
+      // keeps PD of them in flight instead of draining them at a branch
+      const gptr<const double> Wk = k0 + 4 * PD < n ? Wi + (size_t)(k0 + 4 * PD) * n
+                                                    : (i > 0 ? Wi - nn : W);
+#pragma unroll
+      for (int u = 0; u < PD; ++u) {
+        const int k = k0 + 4 * u;
+        const double b0 = Bs[cur][li][k + lk], b1 = Bs[cur][16 + li][k + lk];
+#pragma unroll
+        for (int s = 0; s < RTW; ++s) {
+          if (has[s]) {
+            acc[s][0] = mfma(a[u][s], b0, acc[s][0]);
+            acc[s][1] = mfma(a[u][s], b1, acc[s][1]);
+          }
+        }
+        load(u, Wk + 4 * u * n);
+      }
+    }
+    const bool dz = b == p - 1 && i <= p - 3;
+    const gptr<const double> Zi = glb(t.Z2) + (size_t)i * nn;
+#pragma unroll
+    for (int s = 0; s < RTW; ++s) {
+      if (!has[s]) continue;
+      const int rt = wave + NW * s;
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int col = ct * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rt * 16 + crow(lk, r);
+          double v = acc[s][ct][r];
+          if (dz) v += Zi[(size_t)(c0 + col) * n + row];
+          A[(size_t)(b * n + c0 + col) * lda + i * n + row] = v;
+          Bs[cur ^ 1][col][row] = v;
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // In-place inverse of a <= 32 x 32 upper triangular sub-block (dtrti2).
 // 64 threads; the strictly lower part of the array is left untouched.
 // ---------------------------------------------------------------------------

@@ -64,6 +64,19 @@ struct GemmTask {
   int amask, bmask, dcol0, tiles_n;
 };
 
+// BSTRI's row sweep for block column b >= 1 of one matrix as independent
+// 32-column strips (Item::sub = strip): X_{i,b} = W_i X_{i+1,b} (+ Z2_i for
+// b = p - 1, i <= p - 3) for i = b - 1 .. 0, from the diagonal block X_{b,b}
+// (upper triangular; A holds reflectors below its diagonal).  X_{i,b} is
+// A + (b n) lda + i n; W_i is W + i n n and Z2_i is Z2 + i n n (leading
+// dimension n).
+struct SweepTask {
+  double* A;
+  const double* W;
+  const double* Z2;
+  int lda, n, p, b;
+};
+
 // Inversion of one diagonal sub-block (<= 32 x 32) of an upper triangular
 // block in place.  gidx0 is the 1-based global column index of the
 // sub-block's first diagonal element, reported through *info (atomicMin) if

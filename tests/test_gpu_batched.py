@@ -669,8 +669,10 @@ def test_tuning_knobs_same_bits(B, eps):
     """The BSTRI GEMM tile (32, 64, 128 or the launch-size rule) and the
     reflector slabs per workgroup (1, 2, 4, 8 or the rule) change only speed:
     every value gives the default's bits, for a batch of mixed block sizes
-    whose steps span several tile and slab choices."""
-    sp = [(8, 8), (8, 6), (12, 4), (16, 3), (6, 9), (4, 12)]
+    whose steps span several tile and slab choices.  For n = 256 the rule
+    runs the row sweep as column-strip chains (sweep_kernel) and a forced
+    tile runs it as GEMM rounds: the same bits, over chains of 1 .. 8."""
+    sp = [(8, 8), (8, 6), (12, 4), (16, 3), (6, 9), (4, 12), (16, 9)]
     inputs = [dqmc.dqmc_matrix(lx, p, instance=300 + k)[0] for k, (lx, p) in enumerate(sp)]
     sizes = [(lx * lx, p) for lx, p in sp]
     ref, info = _run_plan(B, inputs, sizes, 0)
