@@ -732,35 +732,49 @@ struct Planner {
     return t;
   }
 
-  // The row sweep of every matrix with 128 <= n <= 256, 32 | n, as one
-  // launch of column-strip chains (sweep_kernel): each strip of block column
-  // b runs its b products in turn, the strips side by side, longest chain
-  // first, and the strips of one block column on one XCD (they read the same
-  // W_i at the same time).  The strips' elements are the GEMM rounds' to the
-  // bit, so which matrices take it never changes a result.  Below n = 128 the
-  // rounds stay: their launches hold many small products at full occupancy,
-  // while a strip's steps are short and latency-bound (the 512-matrix n = 64
-  // batch ran 3.6 % slower through strips).  A matrix whose tuning forces
-  // a GEMM tile keeps the rounds with that tile.  Returns the matrices left.
+  // The row sweep of matrices with 32 | n <= 256 as column-strip chains
+  // (sweep_kernel): each strip of block column b runs its b products in
+  // turn, the strips side by side, longest chain first, and the strips of
+  // one block column on one XCD (they read the same W_i at the same time).
+  // One launch per kernel instance: n <= 64 (4 waves, 4 workgroups per CU),
+  // n <= 128 (8 waves, 2 per CU) and n <= 256 (8 waves, 2 row tiles each,
+  // 1 per CU).  The n <= 128 instances take a group only while its strips
+  // fit about two rounds of residency: a small-n step is short and latency-
+  // bound, and many matrices' rounds fill the GPU better (the 512-matrix
+  // n = 64 batch ran 0.7 % slower, the uneven batch 2.7 %, through strips;
+  // one n = 64, p = 32 matrix 3 % faster).  The strips' elements are the
+  // GEMM rounds' to the bit, so which matrices take them never changes a
+  // result; a matrix whose tuning forces a GEMM tile keeps the rounds with
+  // that tile.  Returns the matrices left for the rounds.
+  static int sweep_variant(int n) { return n <= 64 ? 11 : (n <= 128 ? 12 : 13); }
   std::vector<MatJob*> emit_sweep(std::vector<MatJob>& mats) {
     std::vector<MatJob*> rest;
-    std::vector<SweepTask> st;
-    int rtw = 1;
+    std::map<int, std::vector<MatJob*>> groups;
     for (auto& M : mats) {
       const Geo& g = M.g;
-      if (g.n % 32 != 0 || g.n < 128 || g.n > 256 || g.gemm_tile != 0) {
+      if (g.n % 32 != 0 || g.n > 256 || g.gemm_tile != 0)
         rest.push_back(&M);
+      else
+        groups[sweep_variant(g.n)].push_back(&M);
+    }
+    for (auto& kv : groups) {
+      size_t strips = 0;
+      for (MatJob* M : kv.second) strips += (size_t)(M->g.p - 1) * (M->g.n / 32);
+      const size_t per_cu = kv.first == 11 ? 4 : 2;
+      if (kv.first != 13 && strips > 2 * per_cu * cus) {
+        rest.insert(rest.end(), kv.second.begin(), kv.second.end());
         continue;
       }
-      rtw = std::max(rtw, (g.n + 127) / 128);
-      for (int b = 1; b < g.p; ++b) st.push_back({M.A, M.W, M.Z2, g.lda, g.n, g.p, b});
+      std::vector<SweepTask> st;
+      for (MatJob* M : kv.second)
+        for (int b = 1; b < M->g.p; ++b) st.push_back({M->A, M->W, M->Z2, M->g.lda, M->g.n, M->g.p, b});
+      std::stable_sort(st.begin(), st.end(), [](const SweepTask& x, const SweepTask& y) { return x.b > y.b; });
+      std::vector<Item> items;
+      for (size_t id = 0; id < st.size(); ++id)
+        for (int c = 0; c < st[id].n / 32; ++c) items.push_back({(int)id, c});
+      deal_tasks_to_xcds(items, st.size());
+      emit(F_GEMM, kv.first, st, items);
     }
-    std::stable_sort(st.begin(), st.end(), [](const SweepTask& x, const SweepTask& y) { return x.b > y.b; });
-    std::vector<Item> items;
-    for (size_t id = 0; id < st.size(); ++id)
-      for (int c = 0; c < st[id].n / 32; ++c) items.push_back({(int)id, c});
-    deal_tasks_to_xcds(items, st.size());
-    emit(F_GEMM, 10 + rtw, st, items);
     return rest;
   }
 
@@ -1183,13 +1197,15 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_GEMM:
-      if (L.variant == 11 || L.variant == 12) {
+      if (L.variant >= 11 && L.variant <= 13) {
         auto st = (const SweepTask*)(base + L.task_off);
         auto si = (const Item*)(base + L.item_off);
         if (L.variant == 11)
-          hipLaunchKernelGGL(sweep_kernel<1>, dim3(L.grid), dim3(512), 0, s, st, si);
+          hipLaunchKernelGGL((sweep_kernel<1, 4, 68>), dim3(L.grid), dim3(256), 0, s, st, si);
+        else if (L.variant == 12)
+          hipLaunchKernelGGL((sweep_kernel<1, 8, 132>), dim3(L.grid), dim3(512), 0, s, st, si);
         else
-          hipLaunchKernelGGL(sweep_kernel<2>, dim3(L.grid), dim3(512), 0, s, st, si);
+          hipLaunchKernelGGL((sweep_kernel<2, 8, 260>), dim3(L.grid), dim3(512), 0, s, st, si);
       } else if (L.variant == 2)
         hipLaunchKernelGGL(gemm_kernel<1>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
                            (const Item*)(base + L.item_off));
@@ -1260,7 +1276,7 @@ static bool known_variant(const Launch& L) {
       return (L.variant >= 101 && L.variant <= 104) || (L.variant >= 201 && L.variant <= 204) || L.variant == 108 ||
              L.variant == 116 || (L.variant >= 401 && L.variant <= 404);
     case F_GEMM:
-      return (L.variant >= 0 && L.variant <= 2) || L.variant == 11 || L.variant == 12;
+      return (L.variant >= 0 && L.variant <= 2) || (L.variant >= 11 && L.variant <= 13);
     case F_FLOW:
       return L.variant == 21 || L.variant == 42 || L.variant == 84;
     case F_SCAN:

@@ -1606,15 +1606,18 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
 // column, the diagonal block read through the same upper-triangular mask;
 // so the bits are the same.  The planner deals the strips of one block
 // column to one XCD: they start together and step through the same W_i,
-// which each XCD's L2 then fetches once.  Wave w holds row tiles w, w + 8 (RTW of them:
-// n <= 128 RTW, 32 | n) and both 16-column tiles of the strip.  The strip
-// is double-buffered: step i reads one buffer and writes its result to the
-// other, one barrier a step; the 2 x 65 KiB also keep one workgroup per CU,
-// so that a long chain never shares its CU's matrix cores.
-template <int RTW>
-__global__ __launch_bounds__(512) void sweep_kernel(const SweepTask* __restrict__ tasks,
-                                                    const Item* __restrict__ items) {
-  constexpr int NW = 8, SLD = 260, PD = 8;  // PD: k-steps of W_i in flight
+// which each XCD's L2 then fetches once.  NW waves; wave w holds row tiles
+// w, w + NW (RTW of them: n <= 16 NW RTW, 32 | n) and both 16-column tiles
+// of the strip.  The strip is double-buffered in LDS with a row stride of
+// SLD >= n doubles (SLD = 4 mod 32, a stride of 8 banks: the fragment reads
+// and the result writes are free of bank conflicts): step i reads one buffer and writes
+// its result to the other, one barrier a step.  For n > 128 the 2 x 65 KiB
+// keep one workgroup per CU, so that a long chain never shares its CU's
+// matrix cores; the n <= 64 and n <= 128 instances fit four and two.
+template <int RTW, int NW, int SLD>
+__global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restrict__ tasks,
+                                                        const Item* __restrict__ items) {
+  constexpr int PD = 8;  // k-steps of W_i in flight
   const Item it = items[blockIdx.x];
   const SweepTask t = tasks[it.task];
   const int tid = threadIdx.x, lane = tid & 63;

@@ -55,6 +55,9 @@ PHASE_CASES = [("dqmc", 2, 2), ("dqmc", 2, 3), ("dqmc", 2, 4), ("dqmc", 3, 5), (
                ("rand_eye", 300, 3), ("dqmc", 20, 2),
                # 8 | n in [128, 256]: the 4x4x4 scan by default
                ("rand_eye", 128, 3), ("rand_full", 160, 2),
+               # BSTRI's row sweep by column strips with a partial last row-tile pair (n = 224: 14 tiles
+               # over 8 waves) and chains of 1 .. 4
+               ("rand_eye", 224, 5),
                # n = 144, p = 16: BSOI's Q_k formation steps hold more slabs than the GPU has CUs, so a
                # single-matrix plan takes the multi-slab reflector form
                ("dqmc", 12, 16)]
