@@ -732,8 +732,9 @@ struct Planner {
     return t;
   }
 
-  // The row sweep of matrices with 32 | n <= 256 as column-strip chains
-  // (sweep_kernel): each strip of block column b runs its b products in
+  // The row sweep of matrices with n <= 256 as column-strip chains
+  // (sweep_kernel; masked instances where 32 does not divide n, the last
+  // strip partial): each strip of block column b runs its b products in
   // turn, the strips side by side, longest chain first, and the strips of
   // one block column on one XCD (they read the same W_i at the same time).
   // One launch per kernel instance: n <= 64 (4 waves, 4 workgroups per CU),
@@ -746,22 +747,23 @@ struct Planner {
   // GEMM rounds' to the bit, so which matrices take them never changes a
   // result; a matrix whose tuning forces a GEMM tile keeps the rounds with
   // that tile.  Returns the matrices left for the rounds.
-  static int sweep_variant(int n) { return n <= 64 ? 11 : (n <= 128 ? 12 : 13); }
+  static int sweep_variant(int n) { return (n % 32 ? 20 : 10) + (n <= 64 ? 1 : (n <= 128 ? 2 : 3)); }
   std::vector<MatJob*> emit_sweep(std::vector<MatJob>& mats) {
     std::vector<MatJob*> rest;
     std::map<int, std::vector<MatJob*>> groups;
     for (auto& M : mats) {
       const Geo& g = M.g;
-      if (g.n % 32 != 0 || g.n > 256 || g.gemm_tile != 0)
+      if (g.n > 256 || g.gemm_tile != 0)
         rest.push_back(&M);
       else
         groups[sweep_variant(g.n)].push_back(&M);
     }
     for (auto& kv : groups) {
       size_t strips = 0;
-      for (MatJob* M : kv.second) strips += (size_t)(M->g.p - 1) * (M->g.n / 32);
-      const size_t per_cu = kv.first == 11 ? 4 : 2;
-      if (kv.first != 13 && strips > 2 * per_cu * cus) {
+      for (MatJob* M : kv.second) strips += (size_t)(M->g.p - 1) * ((M->g.n + 31) / 32);
+      const int size_class = kv.first % 10;  // 1: n <= 64, 2: n <= 128, 3: n <= 256
+      const size_t per_cu = size_class == 1 ? 4 : 2;
+      if (size_class != 3 && strips > 2 * per_cu * cus) {
         rest.insert(rest.end(), kv.second.begin(), kv.second.end());
         continue;
       }
@@ -771,7 +773,7 @@ struct Planner {
       std::stable_sort(st.begin(), st.end(), [](const SweepTask& x, const SweepTask& y) { return x.b > y.b; });
       std::vector<Item> items;
       for (size_t id = 0; id < st.size(); ++id)
-        for (int c = 0; c < st[id].n / 32; ++c) items.push_back({(int)id, c});
+        for (int c = 0; c < (st[id].n + 31) / 32; ++c) items.push_back({(int)id, c});
       deal_tasks_to_xcds(items, st.size());
       emit(F_GEMM, kv.first, st, items);
     }
@@ -1197,15 +1199,17 @@ static void run_launch(const Launch& L, const char* base, hipStream_t s) {
       break;
     }
     case F_GEMM:
-      if (L.variant >= 11 && L.variant <= 13) {
+      if (L.variant >= 11) {
         auto st = (const SweepTask*)(base + L.task_off);
         auto si = (const Item*)(base + L.item_off);
-        if (L.variant == 11)
-          hipLaunchKernelGGL((sweep_kernel<1, 4, 68>), dim3(L.grid), dim3(256), 0, s, st, si);
-        else if (L.variant == 12)
-          hipLaunchKernelGGL((sweep_kernel<1, 8, 132>), dim3(L.grid), dim3(512), 0, s, st, si);
-        else
-          hipLaunchKernelGGL((sweep_kernel<2, 8, 260>), dim3(L.grid), dim3(512), 0, s, st, si);
+        switch (L.variant) {
+          case 11: hipLaunchKernelGGL((sweep_kernel<1, 4, 68, false>), dim3(L.grid), dim3(256), 0, s, st, si); break;
+          case 12: hipLaunchKernelGGL((sweep_kernel<1, 8, 132, false>), dim3(L.grid), dim3(512), 0, s, st, si); break;
+          case 13: hipLaunchKernelGGL((sweep_kernel<2, 8, 260, false>), dim3(L.grid), dim3(512), 0, s, st, si); break;
+          case 21: hipLaunchKernelGGL((sweep_kernel<1, 4, 68, true>), dim3(L.grid), dim3(256), 0, s, st, si); break;
+          case 22: hipLaunchKernelGGL((sweep_kernel<1, 8, 132, true>), dim3(L.grid), dim3(512), 0, s, st, si); break;
+          default: hipLaunchKernelGGL((sweep_kernel<2, 8, 260, true>), dim3(L.grid), dim3(512), 0, s, st, si); break;
+        }
       } else if (L.variant == 2)
         hipLaunchKernelGGL(gemm_kernel<1>, dim3(L.grid), dim3(256), 0, s, (const GemmTask*)(base + L.task_off),
                            (const Item*)(base + L.item_off));
@@ -1276,7 +1280,8 @@ static bool known_variant(const Launch& L) {
       return (L.variant >= 101 && L.variant <= 104) || (L.variant >= 201 && L.variant <= 204) || L.variant == 108 ||
              L.variant == 116 || (L.variant >= 401 && L.variant <= 404);
     case F_GEMM:
-      return (L.variant >= 0 && L.variant <= 2) || (L.variant >= 11 && L.variant <= 13);
+      return (L.variant >= 0 && L.variant <= 2) || (L.variant >= 11 && L.variant <= 13) ||
+             (L.variant >= 21 && L.variant <= 23);
     case F_FLOW:
       return L.variant == 21 || L.variant == 42 || L.variant == 84;
     case F_SCAN:

@@ -1604,7 +1604,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
 // Every element is gemm_kernel's: one v_mfma_f64_16x16x4f64 chain over k in
 // steps of 4, increasing, times alpha = 1, plus Z2_i on the last block
 // column, the diagonal block read through the same upper-triangular mask;
-// so the bits are the same.  The planner deals the strips of one block
+// so the bits are the same (for n not a multiple of 32, MASK, rows, columns
+// and k beyond n enter as zeros, as in gemm_kernel).  The planner deals the strips of one block
 // column to one XCD: they start together and step through the same W_i,
 // which each XCD's L2 then fetches once.  NW waves; wave w holds row tiles
 // w, w + NW (RTW of them: n <= 16 NW RTW, 32 | n) and both 16-column tiles
@@ -1614,7 +1615,7 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(const GemmTask* __restrict
 // its result to the other, one barrier a step.  For n > 128 the 2 x 65 KiB
 // keep one workgroup per CU, so that a long chain never shares its CU's
 // matrix cores; the n <= 64 and n <= 128 instances fit four and two.
-template <int RTW, int NW, int SLD>
+template <int RTW, int NW, int SLD, bool MASK>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restrict__ tasks,
                                                         const Item* __restrict__ items) {
   constexpr int PD = 8;  // k-steps of W_i in flight
@@ -1625,32 +1626,51 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restr
   const int li = lane & 15, lk = lane >> 4;
   const int n = t.n, p = t.p, lda = t.lda;
   const int b = t.b, c0 = it.sub * 32;
+  // k runs to n32 in chunks of 4 PD: past n both operands are zero, exact
+  // no-ops on the accumulators (which are never -0), as in gemm_kernel's
+  // zero-padded last k chunk
+  const int n32 = MASK ? (n + 31) & ~31 : n;
   __shared__ double Bs[2][32][SLD];  // B[k][col] at Bs[buf][col][k]
   const gptr<double> A = glb(t.A);
-  for (int e = tid; e < 32 * n; e += NW * 64) {
-    const int k = e % n, col = e / n;
-    Bs[0][col][k] = (k <= c0 + col) ? A[(size_t)(b * n + c0 + col) * lda + b * n + k] : 0.0;
+  for (int e = tid; e < 32 * n32; e += NW * 64) {
+    const int k = e % n32, col = e / n32, gc = c0 + col;
+    Bs[0][col][k] = (gc < n && k <= gc) ? A[(size_t)(b * n + gc) * lda + b * n + k] : 0.0;
+    if (MASK) Bs[1][col][k] = 0.0;  // rows the results never reach stay zero
   }
-  // Row tiles of this wave (uniform), and each lane's offset into W_i: a
-  // tile beyond n reads tile 0's rows (in bounds) and is never multiplied.
-  bool has[RTW];
-  int woff[RTW];
+  // Row tiles of this wave (uniform), and each lane's row of W_i: a tile
+  // beyond n reads tile 0's rows (in bounds) and is never multiplied; a row
+  // beyond n (MASK) reads row n - 1 and takes zero.
+  bool has[RTW], rok[RTW];
+  int wrow[RTW];
 #pragma unroll
   for (int s = 0; s < RTW; ++s) {
     const int rt = wave + NW * s;
     has[s] = rt * 16 < n;
-    woff[s] = lk * n + (has[s] ? rt * 16 : 0) + li;
+    const int row = (has[s] ? rt * 16 : 0) + li;
+    rok[s] = row < n;
+    wrow[s] = MASK ? min(row, n - 1) : row;
   }
   // A fragments W_i[rt 16 + li][k + lk], PD k-steps ahead across steps
   const gptr<const double> W = glb(t.W);
   const size_t nn = (size_t)n * n;
+  // (MASK: the zero for a row or k beyond n is selected when the fragment
+  // is used, not when it arrives, so that the loads stay in flight)
   double a[PD][RTW];
-  auto load = [&](int u, const gptr<const double> Wk) {
+  bool am[PD][RTW];
+  auto load = [&](int u, const gptr<const double> Wb, int k) {
+    const int kk = k + lk;
 #pragma unroll
-    for (int s = 0; s < RTW; ++s) a[u][s] = Wk[woff[s]];
+    for (int s = 0; s < RTW; ++s) {
+      if (MASK) {
+        a[u][s] = Wb[(size_t)min(kk, n - 1) * n + wrow[s]];
+        am[u][s] = kk < n && rok[s];
+      } else {
+        a[u][s] = Wb[(size_t)kk * n + wrow[s]];
+      }
+    }
   };
 #pragma unroll
-  for (int u = 0; u < PD; ++u) load(u, W + (size_t)(b - 1) * nn + 4 * u * n);
+  for (int u = 0; u < PD; ++u) load(u, W + (size_t)(b - 1) * nn, 4 * u);
   __syncthreads();
   int cur = 0;
   for (int i = b - 1; i >= 0; --i) {
@@ -1658,13 +1678,14 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restr
     d4 acc[RTW][2];
 #pragma unroll
     for (int s = 0; s < RTW; ++s) acc[s][0] = acc[s][1] = d4zero();
-    for (int k0 = 0; k0 < n; k0 += 4 * PD) {
+    for (int k0 = 0; k0 < n32; k0 += 4 * PD) {
       // the chunk the slots load next: the rest of W_i, then the start of
-      // W_{i-1} (4 PD | n); after W_0 they reload W_0's first rows (in
-      // bounds, never used): the loads stay unconditional, so the compiler
-      // keeps PD of them in flight instead of draining them at a branch
-      const gptr<const double> Wk = k0 + 4 * PD < n ? Wi + (size_t)(k0 + 4 * PD) * n
-                                                    : (i > 0 ? Wi - nn : W);
+      // W_{i-1}; after W_0 they reload W_0's first rows (in bounds, never
+      // used): the loads stay unconditional, so the compiler keeps PD of
+      // them in flight instead of draining them at a branch
+      const bool in_i = k0 + 4 * PD < n32;
+      const gptr<const double> Wb = in_i ? Wi : (i > 0 ? Wi - nn : W);
+      const int kb = in_i ? k0 + 4 * PD : 0;
 #pragma unroll
       for (int u = 0; u < PD; ++u) {
         const int k = k0 + 4 * u;
@@ -1672,11 +1693,12 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restr
 #pragma unroll
         for (int s = 0; s < RTW; ++s) {
           if (has[s]) {
-            acc[s][0] = mfma(a[u][s], b0, acc[s][0]);
-            acc[s][1] = mfma(a[u][s], b1, acc[s][1]);
+            const double av = MASK ? (am[u][s] ? a[u][s] : 0.0) : a[u][s];
+            acc[s][0] = mfma(av, b0, acc[s][0]);
+            acc[s][1] = mfma(av, b1, acc[s][1]);
           }
         }
-        load(u, Wk + 4 * u * n);
+        load(u, Wb, kb + 4 * u);
       }
     }
     const bool dz = b == p - 1 && i <= p - 3;
@@ -1691,9 +1713,11 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restr
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = rt * 16 + crow(lk, r);
+          // a row or column beyond n holds an exact zero (zero operands)
+          const bool ok = !MASK || (c0 + col < n && row < n);
           double v = acc[s][ct][r];
-          if (dz) v += Zi[(size_t)(c0 + col) * n + row];
-          A[(size_t)(b * n + c0 + col) * lda + i * n + row] = v;
+          if (dz && ok) v += Zi[(size_t)(c0 + col) * n + row];
+          if (ok) A[(size_t)(b * n + c0 + col) * lda + i * n + row] = v;
           Bs[cur ^ 1][col][row] = v;
         }
       }

@@ -65,7 +65,8 @@ struct GemmTask {
 };
 
 // BSTRI's row sweep for block column b >= 1 of one matrix as independent
-// 32-column strips (Item::sub = strip): X_{i,b} = W_i X_{i+1,b} (+ Z2_i for
+// 32-column strips (Item::sub = strip; the last one partial where 32 does
+// not divide n): X_{i,b} = W_i X_{i+1,b} (+ Z2_i for
 // b = p - 1, i <= p - 3) for i = b - 1 .. 0, from the diagonal block X_{b,b}
 // (upper triangular; A holds reflectors below its diagonal).  X_{i,b} is
 // A + (b n) lda + i n; W_i is W + i n n and Z2_i is Z2 + i n n (leading
