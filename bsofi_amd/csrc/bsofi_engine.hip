@@ -736,7 +736,8 @@ struct Planner {
   // (sweep_kernel; masked instances where 32 does not divide n, the last
   // strip partial): each strip of block column b runs its b products in
   // turn, the strips side by side, longest chain first, and the strips of
-  // one block column on one XCD (they read the same W_i at the same time).
+  // one block column on one XCD (they read the same W_i at the same time),
+  // the block columns spread over the XCDs by work.
   // One launch per kernel instance: n <= 64 (4 waves, 4 workgroups per CU),
   // n <= 128 (8 waves, 2 per CU) and n <= 256 (8 waves, 2 row tiles each,
   // 1 per CU).  The n <= 128 instances take a group only while its strips
@@ -771,10 +772,29 @@ struct Planner {
       for (MatJob* M : kv.second)
         for (int b = 1; b < M->g.p; ++b) st.push_back({M->A, M->W, M->Z2, M->g.lda, M->g.n, M->g.p, b});
       std::stable_sort(st.begin(), st.end(), [](const SweepTask& x, const SweepTask& y) { return x.b > y.b; });
+      // Block columns to XCDs, longest first, each to the XCD with the least
+      // work so far (b n^2 per strip): round-robin dealing left one XCD 70
+      // steps of strips for C2's 63-step critical path, this gives 63.
+      // Workgroup i runs on XCD i mod 8, so the XCDs' queues are padded to
+      // one length with an empty task (b = 0, which returns at once) and
+      // then interleaved.
+      constexpr int kXcd = 8;
+      std::vector<std::vector<Item>> q(kXcd);
+      std::vector<double> load(kXcd, 0.0);
+      for (size_t id = 0; id < st.size(); ++id) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const int strips = (st[id].n + 31) / 32;
+        for (int c = 0; c < strips; ++c) q[x].push_back({(int)id, c});
+        load[x] += (double)st[id].b * strips * st[id].n * st[id].n;
+      }
+      size_t len = 0;
+      for (auto& v : q) len = std::max(len, v.size());
+      bool pad = false;
+      for (auto& v : q) pad = pad || v.size() < len;
+      if (pad) st.push_back({st[0].A, st[0].W, st[0].Z2, st[0].lda, st[0].n, st[0].p, 0});
       std::vector<Item> items;
-      for (size_t id = 0; id < st.size(); ++id)
-        for (int c = 0; c < (st[id].n + 31) / 32; ++c) items.push_back({(int)id, c});
-      deal_tasks_to_xcds(items, st.size());
+      for (size_t r = 0; r < len; ++r)
+        for (int x = 0; x < kXcd; ++x) items.push_back(r < q[x].size() ? q[x][r] : Item{(int)st.size() - 1, 0});
       emit(F_GEMM, kv.first, st, items);
     }
     return rest;

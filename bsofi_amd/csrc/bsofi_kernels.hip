@@ -1621,6 +1621,7 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepTask* __restr
   constexpr int PD = 8;  // k-steps of W_i in flight
   const Item it = items[blockIdx.x];
   const SweepTask t = tasks[it.task];
+  if (t.b <= 0) return;  // an empty task (the planner's padding)
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lk = lane >> 4;

@@ -70,7 +70,7 @@ struct GemmTask {
 // b = p - 1, i <= p - 3) for i = b - 1 .. 0, from the diagonal block X_{b,b}
 // (upper triangular; A holds reflectors below its diagonal).  X_{i,b} is
 // A + (b n) lda + i n; W_i is W + i n n and Z2_i is Z2 + i n n (leading
-// dimension n).
+// dimension n).  A task with b = 0 is empty.
 struct SweepTask {
   double* A;
   const double* W;

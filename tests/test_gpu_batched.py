@@ -386,6 +386,36 @@ def test_tuned_variants_match_oracle(B, eps):
     ctx.close()
 
 
+def test_scan_forms_same_bits(B):
+    """Every scan form the autotuner may pick for a sub-panel width gives the
+    same bits: the 16-, 32- and 64-row tiles on the 16x16x4 MFMA and the
+    32-row 4x4x4 forms (one or two workgroups per CU) run each element's k
+    in the same order of 4-term steps, and the two FP64 MFMAs evaluate a
+    step the same way."""
+    import torch
+
+    from bsofi_amd import autotune
+
+    ctx = B.Context(0)
+    for lx, p in [(8, 4), (10, 3), (12, 3), (16, 3)]:
+        n = lx * lx
+        A0, _ = dqmc.dqmc_matrix(lx, p, instance=70 + lx)
+        res = {}
+        for nb, mti in autotune.candidates(n):
+            print(f"scan form case n={n} p={p} nb={nb} scan_mti={mti}", flush=True)
+            ctx.set_tuning(n, nb, mti)
+            d = to_dev(A0)
+            t = torch.zeros(n * p, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            assert B.invert_batched(ctx, [(d.data_ptr(), t.data_ptr(), n, p, n * p)]) == [0]
+            res[(nb, mti)] = from_dev(d, n * p)
+        for (nb, mti), got in res.items():
+            ref = res[min(k for k in res if k[0] == nb)]
+            assert np.array_equal(got, ref), (n, nb, mti)
+        ctx.clear_tuning()
+    ctx.close()
+
+
 def test_scan4_buffer_depths_same_bits(B):
     """The 4x4x4 scan with operands loaded one (scan_mti 9) or two (10)
     k-steps ahead, and the latter with two workgroups per CU (12, n <= 64),
